@@ -1,0 +1,24 @@
+"""ifgf_amd: IFGF operator application on AMD MI355X (gfx950).
+
+A from-scratch HIP implementation of TimHRO/ifgf's IfgfOperator init/mult path behind the
+reference's `pyifgf` API (see ifgf_amd.pyifgf) and a C ABI (include/ifgf_amd.h).
+"""
+from .pyifgf import (  # noqa: F401
+    CombinedFieldHelmholtzIfgfOperator,
+    DoubleLayerHelmholtzIfgfOperator,
+    GradHelmholtzIfgfOperator,
+    HelmholtzIfgfOperator,
+    LaplaceIfgfOperator,
+    MofifiedHelmholtzIfgfOperator,
+    ModifiedHelmholtzIfgfOperator,
+)
+
+__all__ = [
+    "HelmholtzIfgfOperator",
+    "MofifiedHelmholtzIfgfOperator",
+    "ModifiedHelmholtzIfgfOperator",
+    "GradHelmholtzIfgfOperator",
+    "LaplaceIfgfOperator",
+    "DoubleLayerHelmholtzIfgfOperator",
+    "CombinedFieldHelmholtzIfgfOperator",
+]

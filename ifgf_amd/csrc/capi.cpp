@@ -1,0 +1,271 @@
+// capi.cpp -- the extern "C" boundary (include/ifgf_amd.h) over the plan and the device
+// pipeline.  Host C++ only; the HIP code lives in device.hip.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "../../include/ifgf_amd.h"
+#include "device.hpp"
+#include "plan.hpp"
+
+using namespace ifgf;
+
+struct ifgf_operator
+{
+    KernelSpec K;
+    int64_t leaf = 64;
+    int order = 6;
+    int n_elem = 1;
+    int precision = PREC_MIXED;
+    bool planned = false;
+    bool on_device = false;
+    Plan plan;
+    DevicePlan dev;
+    std::string err;
+};
+
+static int fail(ifgf_operator* op, int code, const std::string& msg)
+{
+    if (op)
+        op->err = msg;
+    return code;
+}
+
+extern "C"
+{
+    const char* ifgf_version(void) { return "ifgf_amd 0.1 (gfx950)"; }
+
+    int ifgf_device_count(void)
+    {
+        return device_available() ? 1 : 0;
+    }
+
+    int ifgf_create(int kernel, double k_re, double k_im, int64_t leaf_size, int order, int n_elem,
+                    double tol, int precision, ifgf_operator** out)
+    {
+        if (!out)
+            return IFGF_EINVAL;
+        *out = nullptr;
+        if (kernel < IFGF_HELMHOLTZ || kernel > IFGF_GRAD || order < 1 || n_elem < 1 ||
+            (precision != IFGF_PREC_FP64 && precision != IFGF_PREC_MIXED))
+            return IFGF_EINVAL;
+        auto* op = new (std::nothrow) ifgf_operator();
+        if (!op)
+            return IFGF_EINTERNAL;
+        op->K.kind = kernel;
+        op->K.tol = tol;
+        op->K.kappa = k_re;
+        if (kernel == IFGF_HELMHOLTZ || kernel == IFGF_COMBINED)
+            op->K.k = cplx(0.0, -k_re); // e^{-k d} = e^{i kappa d}
+        else if (kernel == IFGF_LAPLACE)
+            op->K.k = cplx(0.0, 0.0);
+        else
+            op->K.k = cplx(k_re, k_im);
+        op->leaf = leaf_size;
+        op->order = order;
+        op->n_elem = n_elem;
+        op->precision = precision;
+        *out = op;
+        return IFGF_OK;
+    }
+
+    int ifgf_set_dx(ifgf_operator* op, int dx)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (op->K.kind != IFGF_GRAD || dx < -1 || dx > 2)
+            return fail(op, IFGF_EINVAL, "setDx applies to the gradient kernel, dx in -1..2");
+        op->K.dx = dx;
+        op->plan.K.dx = dx;
+        op->dev.dx = dx;
+        return IFGF_OK;
+    }
+
+    int ifgf_init(ifgf_operator* op, const double* srcs, int64_t n_src, const double* tgts,
+                  int64_t n_tgt, const double* normals, ifgf_cutoff_fn cutoff, void* cutoff_ctx,
+                  int flags)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!srcs || !tgts || n_src <= 0 || n_tgt <= 0)
+            return fail(op, IFGF_EINVAL, "init needs non-empty source and target arrays");
+        if ((op->K.kind == IFGF_DOUBLELAYER || op->K.kind == IFGF_COMBINED) && !normals)
+            return fail(op, IFGF_EINVAL, "this kernel needs source normals");
+        const bool plan_only = (flags & IFGF_INIT_PLAN_ONLY) != 0;
+        if (!plan_only && !device_available())
+            return fail(op, IFGF_ENODEV, "no HIP device visible: ifgf_amd has no CPU path");
+        try
+        {
+            device_free(op->dev);
+            op->on_device = false;
+            op->planned = false;
+            op->plan = Plan();
+            op->plan.K = op->K;
+            std::function<bool(double)> cb;
+            if (cutoff)
+                cb = [cutoff, cutoff_ctx](double d) { return cutoff(d, cutoff_ctx) != 0; };
+            std::string e = build_plan(op->plan, srcs, n_src, tgts, n_tgt, normals, op->leaf,
+                                       op->order, op->n_elem, cb, default_threads());
+            if (!e.empty())
+                return fail(op, IFGF_EINVAL, e);
+            op->planned = true;
+            if (plan_only)
+                return IFGF_OK;
+            e = device_upload(op->plan, op->precision, op->dev);
+            if (!e.empty())
+                return fail(op, IFGF_EINTERNAL, e);
+            op->on_device = true;
+        }
+        catch (const std::exception& ex)
+        {
+            return fail(op, IFGF_EINTERNAL, ex.what());
+        }
+        return IFGF_OK;
+    }
+
+    int ifgf_mult(ifgf_operator* op, const double* w, int64_t nrhs, double* out, int flags,
+                  void* stream)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!op->on_device)
+            return fail(op, IFGF_ESTATE, "mult before a device init");
+        if (!w || !out || nrhs < 1)
+            return fail(op, IFGF_EINVAL, "mult needs weights, output and nrhs >= 1");
+        std::string e = device_mult(op->dev, w, out, nrhs, (flags & IFGF_DEVICE_POINTERS) != 0,
+                                    stream);
+        if (!e.empty())
+            return fail(op, IFGF_EINTERNAL, e);
+        return IFGF_OK;
+    }
+
+    void ifgf_destroy(ifgf_operator* op)
+    {
+        if (!op)
+            return;
+        device_free(op->dev);
+        delete op;
+    }
+
+    const char* ifgf_last_error(const ifgf_operator* op) { return op ? op->err.c_str() : "null operator"; }
+
+    int ifgf_plan_counts(const ifgf_operator* op, int64_t* c)
+    {
+        if (!op || !c)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        const Plan& P = op->plan;
+        c[0] = P.S.levels;
+        c[1] = P.S.num_boxes();
+        c[2] = P.T.levels;
+        c[3] = P.T.num_boxes();
+        c[4] = int64_t(P.far.size());
+        c[5] = int64_t(P.near.size());
+        c[6] = P.num_cones();
+        c[7] = P.n_fine;
+        c[8] = P.n_fine_uncovered;
+        c[9] = P.n_tofar;
+        c[10] = op->dev.bytes_total;
+        c[11] = int64_t(P.fine_ids.size());
+        return IFGF_OK;
+    }
+
+    int ifgf_plan_orders(const ifgf_operator* op, int* o)
+    {
+        if (!op || !o)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        for (int d = 0; d < 3; d++)
+        {
+            o[d] = op->plan.base_order[d];
+            o[3 + d] = op->plan.ho_order[d];
+            o[6 + d] = op->plan.lo_order[d];
+            o[9 + d] = op->plan.factor[d];
+        }
+        return IFGF_OK;
+    }
+
+    int ifgf_plan_tree(const ifgf_operator* op, int tree, int64_t* level_off, int64_t* begin,
+                       int64_t* end, int32_t* parent, int8_t* leaf, double* bbox6, int64_t* perm)
+    {
+        if (!op || (tree != 0 && tree != 1))
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        const Tree& T = tree == 0 ? op->plan.S : op->plan.T;
+        const int64_t nb = T.num_boxes();
+        if (level_off)
+            std::memcpy(level_off, T.level_off.data(), T.level_off.size() * sizeof(int64_t));
+        if (begin)
+            std::memcpy(begin, T.begin.data(), nb * sizeof(int64_t));
+        if (end)
+            std::memcpy(end, T.end.data(), nb * sizeof(int64_t));
+        if (parent)
+            std::memcpy(parent, T.parent.data(), nb * sizeof(int32_t));
+        if (leaf)
+            for (int64_t g = 0; g < nb; g++)
+                leaf[g] = int8_t(T.leaf[g]);
+        if (bbox6)
+            for (int64_t g = 0; g < nb; g++)
+                for (int d = 0; d < 3; d++)
+                {
+                    bbox6[6 * g + d] = T.blo[3 * g + d];
+                    bbox6[6 * g + 3 + d] = T.bhi[3 * g + d];
+                }
+        if (perm)
+            std::memcpy(perm, T.perm.data(), T.perm.size() * sizeof(int64_t));
+        return IFGF_OK;
+    }
+
+    int ifgf_plan_pairs(const ifgf_operator* op, int far, int32_t* src, int32_t* tgt)
+    {
+        if (!op || !src || !tgt)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        const auto& v = far ? op->plan.far : op->plan.near;
+        for (size_t i = 0; i < v.size(); i++)
+        {
+            src[i] = v[i].src;
+            tgt[i] = v[i].tgt;
+        }
+        return IFGF_OK;
+    }
+
+    int ifgf_plan_cones(const ifgf_operator* op, int64_t* nc, int64_t* nf, int64_t* cone_off,
+                        int64_t* cone_cell, uint32_t* mask, int64_t* fine_off, int64_t* fine_ids)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        const Plan& P = op->plan;
+        const int64_t nb = P.S.num_boxes();
+        if (nc)
+            std::memcpy(nc, P.nc.data(), 3 * nb * sizeof(int64_t));
+        if (nf)
+            std::memcpy(nf, P.nf.data(), 3 * nb * sizeof(int64_t));
+        if (cone_off)
+            std::memcpy(cone_off, P.cone_off.data(), (nb + 1) * sizeof(int64_t));
+        if (cone_cell)
+            std::memcpy(cone_cell, P.cone_cell.data(), P.cone_cell.size() * sizeof(int64_t));
+        if (mask)
+            std::memcpy(mask, P.cone_mask.data(), P.cone_mask.size() * sizeof(uint32_t));
+        if (fine_off)
+            std::memcpy(fine_off, P.fine_off.data(), (nb + 1) * sizeof(int64_t));
+        if (fine_ids)
+            std::memcpy(fine_ids, P.fine_ids.data(), P.fine_ids.size() * sizeof(int64_t));
+        return IFGF_OK;
+    }
+
+    int ifgf_last_timings(const ifgf_operator* op, double* ms6)
+    {
+        if (!op || !ms6)
+            return IFGF_EINVAL;
+        device_last_timings(op->dev, ms6);
+        return IFGF_OK;
+    }
+}

@@ -1,0 +1,109 @@
+// device.hpp -- device-resident plan and the mult pipeline (host-side declarations).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace ifgf
+{
+enum Precision
+{
+    PREC_FP64 = 0,  // every step in FP64 (the plain form)
+    PREC_MIXED = 1  // FP32 arithmetic; FP64 for cone coordinates, phases and box geometry
+};
+
+// geometry and cone grid of one source box, as read by the kernels
+struct BoxGeo
+{
+    double ctr[3];
+    double H;
+    double lo[3];  // cone domain (s, theta, phi)
+    double hi[3];
+    double hf[3];  // fine cell widths
+    double hc[3];  // coarse cell widths
+    int32_t nf[3]; // fine cells per axis
+    int32_t nc[3]; // coarse cells per axis
+    int64_t table_off;
+};
+
+struct ConeDev
+{
+    int32_t box;
+    int32_t cell;      // coarse cell id within the box grid
+    uint32_t mask;     // active fine sub-cones
+    int32_t fine_base; // first fine slot
+};
+
+struct WorkItem // a chunk of at most 64 consecutive target points of one target box
+{
+    int32_t box;   // target box (global id in the target tree)
+    int32_t first; // first target point (sorted order)
+    int32_t count; // 1..64
+    int32_t pad;
+};
+
+struct DevicePlan
+{
+    int prec = PREC_MIXED;
+    int kind = 0;
+    double k_re = 0, k_im = 0; // e^{-k d}
+    double kappa = 0;
+    int dx = -1;
+    int ho[3], lo[3], f[3];
+    int n_hi = 0, n_lo = 0, nsub = 0;
+    int64_t nsrc = 0, ntgt = 0;
+    int levels = 0;
+    int64_t n_cones = 0, n_fine = 0;
+
+    // device arrays (raw pointers owned by this struct)
+    double* src_pts = nullptr;   // sorted, xyz
+    double* tgt_pts = nullptr;
+    double* normals = nullptr;   // sorted source normals (DL, CF) or null
+    int64_t* src_perm = nullptr;
+    int64_t* tgt_perm = nullptr;
+    int64_t* src_begin = nullptr; // per source box
+    int64_t* src_end = nullptr;
+    int32_t* child = nullptr;     // 8 per source box
+    BoxGeo* geo = nullptr;
+    ConeDev* cones = nullptr;
+    int32_t* table = nullptr;
+    double* ho_nodes = nullptr;   // n_hi x 3 reference nodes in [-1,1]^3
+    void* wmat = nullptr;         // axis maps (R): W0 | W1 | W2
+    int woff[4] = {0, 0, 0, 0};
+
+    int32_t* leaf_cones = nullptr;
+    int64_t n_leaf_cones = 0;
+    int32_t* up_cones = nullptr;             // non-leaf cones, level-major
+    std::vector<int64_t> up_off;             // per level, into up_cones
+    std::vector<int64_t> cone_lvl_off;       // per level, global cone range
+
+    int32_t* far_off = nullptr;   // per target box: CSR into far_src
+    int32_t* far_src = nullptr;
+    int32_t* near_off = nullptr;
+    int32_t* near_src = nullptr;
+    WorkItem* far_items = nullptr;
+    std::vector<int64_t> far_item_off; // per target level
+    WorkItem* near_items = nullptr;
+    int64_t n_near_items = 0;
+
+    // work buffers (resized per nrhs)
+    void* wsorted = nullptr;  // N complex (R)
+    void* result = nullptr;   // M complex (R)
+    void* V = nullptr;        // n_cones * n_hi complex (R)
+    void* F = nullptr;        // n_fine * n_lo complex (R)
+    int64_t bytes_total = 0;
+};
+
+std::string device_upload(const Plan& P, int precision, DevicePlan& D);
+void device_free(DevicePlan& D);
+// w: complex128 (host or device), out: complex128 (host or device); nrhs columns of length
+// N / M stored one after another.
+std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
+                        void* stream);
+// per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
+void device_last_timings(const DevicePlan& D, double* out6);
+bool device_available();
+} // namespace ifgf

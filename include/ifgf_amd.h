@@ -1,0 +1,101 @@
+/* ifgf_amd.h -- C ABI of the MI355X IFGF operator library (libifgf_amd.so).
+ *
+ * Drop-in boundary for TimHRO/ifgf's Python module `pyifgf` (py_ifgf.cpp:16-48): one opaque
+ * operator per IfgfOperator instance, with the reference's construct / init / mult
+ * semantics.  Plain pointers and sizes only; no exception crosses the ABI: every call
+ * returns IFGF_OK or a negative code, and ifgf_last_error() describes the failure.
+ *
+ * Points are xyz-interleaved float64 arrays (N x 3, i.e. the column-major 3 x N
+ * Eigen::Array<double,3,Dynamic> the reference takes).  Densities and results are
+ * complex128 (interleaved re, im); nrhs density vectors are stored one after another.
+ */
+#ifndef IFGF_AMD_H
+#define IFGF_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFGF_OK 0
+#define IFGF_EINVAL (-1)    /* bad argument */
+#define IFGF_ENODEV (-2)    /* no HIP device: the library has no CPU path */
+#define IFGF_ESTATE (-3)    /* call out of order (mult before init, ...) */
+#define IFGF_EINTERNAL (-4) /* HIP or allocation failure */
+
+/* kernel families (the L5 classes of the reference) */
+#define IFGF_HELMHOLTZ 0     /* helmholtz_ifgf.hpp: e^{i kappa d}/(4 pi d), k_re = kappa      */
+#define IFGF_MODHELMHOLTZ 1  /* modified_helmholtz_ifgf.hpp: e^{-k d}/(4 pi d), complex k     */
+#define IFGF_LAPLACE 2       /* laplace_ifgf.hpp: 1/(4 pi d)                                  */
+#define IFGF_DOUBLELAYER 3   /* double_layer_helmholtz_ifgf.hpp: complex k, source normals    */
+#define IFGF_COMBINED 4      /* combined_field_helmholtz_ifgf.hpp: kappa = k_re, normals      */
+#define IFGF_GRAD 5          /* grad_helmholtz_ifgf.hpp: complex k, ifgf_set_dx               */
+
+/* arithmetic of mult */
+#define IFGF_PREC_FP64 0  /* every step in FP64 */
+#define IFGF_PREC_MIXED 1 /* FP32 arithmetic; FP64 cone coordinates, phases, geometry */
+
+/* ifgf_init flags */
+#define IFGF_INIT_PLAN_ONLY 1 /* build the host plan only (introspection); mult is refused */
+
+/* ifgf_mult flags */
+#define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */
+
+typedef struct ifgf_operator ifgf_operator;
+
+/* cutOff callback of init (ifgfoperator.hpp:65-67): nonzero drops an admissible pair */
+typedef int (*ifgf_cutoff_fn)(double dist, void* ctx);
+
+/* Replaces the IfgfOperator constructors (ifgfoperator.hpp:36-59; py_ifgf.cpp:19,46):
+ * (k, leafSize, order, n_elem, tolerance).  k_re/k_im: complex wavenumber (k_re = kappa
+ * for the real-kappa families; ignored for Laplace).  leaf_size < 0 means unbounded. */
+int ifgf_create(int kernel, double k_re, double k_im, int64_t leaf_size, int order, int n_elem,
+                double tol, int precision, ifgf_operator** out);
+
+/* GradHelmholtzIfgfOperator::setDx (grad_helmholtz_ifgf.hpp:60): -1 potential, 0..2 d/dy_i */
+int ifgf_set_dx(ifgf_operator* op, int dx);
+
+/* IfgfOperator::init (ifgfoperator.hpp:65-106) and the normals overload of the
+ * double-layer / combined-field operators (double_layer_helmholtz_ifgf.hpp:22-26).
+ * normals may be NULL; cutoff may be NULL (never drop). */
+int ifgf_init(ifgf_operator* op, const double* srcs, int64_t n_src, const double* tgts,
+              int64_t n_tgt, const double* normals, ifgf_cutoff_fn cutoff, void* cutoff_ctx,
+              int flags);
+
+/* IfgfOperator::mult (ifgfoperator.hpp:246-867) for nrhs density vectors.
+ * stream: a hipStream_t (NULL = default stream).  With host pointers the call returns
+ * after the result is copied back; with IFGF_DEVICE_POINTERS it only enqueues. */
+int ifgf_mult(ifgf_operator* op, const double* w, int64_t nrhs, double* out, int flags,
+              void* stream);
+
+void ifgf_destroy(ifgf_operator* op);
+const char* ifgf_last_error(const ifgf_operator* op);
+const char* ifgf_version(void);
+int ifgf_device_count(void);
+
+/* --- plan introspection (integer parity with Octree / ConeDomain) ---------------------
+ * counts: [0] source levels [1] source boxes [2] target levels [3] target boxes
+ *         [4] far pairs [5] near pairs [6] coarse cones [7] fine cones (covered)
+ *         [8] fine cones active but not covered [9] dropped (cutOff) pairs
+ *         [10] device bytes [11] fine cone ids listed */
+int ifgf_plan_counts(const ifgf_operator* op, int64_t* counts);
+/* orders: base[3], coarse (high) order[3], fine (low) order[3], factor[3] */
+int ifgf_plan_orders(const ifgf_operator* op, int* out12);
+/* tree 0 = sources, 1 = targets; arrays sized by the counts above (level_off: levels+1) */
+int ifgf_plan_tree(const ifgf_operator* op, int tree, int64_t* level_off, int64_t* begin,
+                   int64_t* end, int32_t* parent, int8_t* leaf, double* bbox6, int64_t* perm);
+/* interaction pairs (global source box, global target box), sorted by target then source */
+int ifgf_plan_pairs(const ifgf_operator* op, int far, int32_t* src, int32_t* tgt);
+/* per source box: coarse/fine elements (3 each), cone_off/fine_off (boxes+1); per coarse
+ * cone: cell id and fine sub-cone mask; fine_ids: active fine cells per box, ascending */
+int ifgf_plan_cones(const ifgf_operator* op, int64_t* nc, int64_t* nf, int64_t* cone_off,
+                    int64_t* cone_cell, uint32_t* mask, int64_t* fine_off, int64_t* fine_ids);
+/* per-phase device times of the last mult in ms (near, leaf, upward, transform, far, total);
+ * filled only when the environment variable IFGF_PHASE_TIMING is set */
+int ifgf_last_timings(const ifgf_operator* op, double* ms6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IFGF_AMD_H */

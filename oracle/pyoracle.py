@@ -1,0 +1,234 @@
+"""ctypes wrapper of the CPU oracle (oracle/ifgf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: a from-scratch FP64 restatement of TimHRO/ifgf's
+IfgfOperator init/mult path used as the checker for the HIP library.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+HELMHOLTZ, MODHELMHOLTZ, LAPLACE, DOUBLELAYER, COMBINED, GRAD = range(6)
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        P, D, I, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int64
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [I, D, D, I64, I, I, D, I]
+        L.orc_destroy.argtypes = [P]
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_last_error.argtypes = [P]
+        L.orc_init.argtypes = [P, P, I64, P, I64, P, I]
+        L.orc_mult.argtypes = [P, P, P, I]
+        L.orc_base_order.argtypes = [P, P]
+        L.orc_levels.argtypes = [P, I]
+        L.orc_num_boxes.restype = I64
+        L.orc_num_boxes.argtypes = [P, I, I]
+        L.orc_perm.argtypes = [P, I, P]
+        L.orc_root_box.argtypes = [P, I, P]
+        L.orc_tree_level.argtypes = [P, I, I, P, P, P, P, P]
+        L.orc_num_interactions.restype = I64
+        L.orc_num_interactions.argtypes = [P, I, I]
+        L.orc_interactions.argtypes = [P, I, I, P]
+        L.orc_cone_counts.argtypes = [P, I, I, P, P]
+        L.orc_cone_active.argtypes = [P, I, I, P]
+        L.orc_min_rec.argtypes = [P]
+        L.orc_chebnodes1d.argtypes = [I, P]
+        L.orc_chebtransform.argtypes = [P, P, P]
+        L.orc_clenshaw.argtypes = [P, P, P, I64, P]
+        L.orc_morton_sort.argtypes = [P, I64, P, P]
+        L.orc_morton_less.argtypes = [P, P, P]
+        L.orc_cart_to_interp.argtypes = [P, P, D, P]
+        L.orc_interp_to_cart.argtypes = [P, P, D, P]
+        L.orc_kernel_eval.argtypes = [P, P, P, I64, P]
+        L.orc_direct.argtypes = [P, P, I64, P, P, P, P, I64, P, I]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _pts(a):
+    """(3, N) or (N, 3) array -> contiguous (N, 3) float64 (xyz interleaved)."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 2 and a.shape[0] == 3 and a.shape[1] != 3:
+        a = a.T
+    return np.ascontiguousarray(a)
+
+
+def default_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class Oracle:
+    """CPU FP64 IFGF operator with the reference's construct/init/mult semantics."""
+
+    def __init__(self, kind, k, leaf, order, n_elem=1, tol=-1.0, dx=-1):
+        k = complex(k)
+        self.kind = kind
+        self.h = lib().orc_create(kind, k.real, k.imag, int(leaf), int(order), int(n_elem),
+                                  float(tol), int(dx))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def init(self, srcs, tgts, normals=None, nthreads=None):
+        s = _pts(srcs)
+        t = _pts(tgts)
+        self._srcs, self._tgts = s, t
+        n = None if normals is None else _pts(normals)
+        self._normals = n
+        rc = lib().orc_init(self.h, _p(s), s.shape[0], _p(t), t.shape[0],
+                            None if n is None else _p(n), nthreads or default_threads())
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error(self.h).decode())
+        return self
+
+    def mult(self, w, nthreads=None):
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
+        out = np.zeros(self._tgts.shape[0], dtype=np.complex128)
+        rc = lib().orc_mult(self.h, _p(w), _p(out), nthreads or default_threads())
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error(self.h).decode())
+        return out
+
+    def base_order(self):
+        o = np.zeros(3, dtype=np.int32)
+        lib().orc_base_order(self.h, _p(o))
+        return o
+
+    def levels(self, tree=0):
+        return lib().orc_levels(self.h, tree)
+
+    def num_boxes(self, tree, level):
+        return lib().orc_num_boxes(self.h, tree, level)
+
+    def perm(self, tree=0):
+        n = (self._srcs if tree == 0 else self._tgts).shape[0]
+        p = np.zeros(n, dtype=np.int64)
+        lib().orc_perm(self.h, tree, _p(p))
+        return p
+
+    def root_box(self, tree=0):
+        b = np.zeros(6)
+        lib().orc_root_box(self.h, tree, _p(b))
+        return b
+
+    def tree_level(self, tree, level):
+        n = self.num_boxes(tree, level)
+        begin = np.zeros(n, np.int64)
+        end = np.zeros(n, np.int64)
+        parent = np.zeros(n, np.int32)
+        leaf = np.zeros(n, np.int8)
+        bbox = np.zeros((n, 6))
+        lib().orc_tree_level(self.h, tree, level, _p(begin), _p(end), _p(parent), _p(leaf),
+                             _p(bbox))
+        return dict(begin=begin, end=end, parent=parent, leaf=leaf.astype(bool), bbox=bbox)
+
+    def interactions(self, level, far):
+        n = lib().orc_num_interactions(self.h, level, int(far))
+        rows = np.zeros((n, 3), np.int64)
+        if n:
+            lib().orc_interactions(self.h, level, int(far), _p(rows))
+        return rows
+
+    def cones(self, level, step):
+        nb = self.num_boxes(0, level)
+        nel = np.zeros((nb, 3), np.int64)
+        nact = np.zeros(nb, np.int64)
+        lib().orc_cone_counts(self.h, level, step, _p(nel), _p(nact))
+        ids = np.zeros(int(nact.sum()), np.int64)
+        if ids.size:
+            lib().orc_cone_active(self.h, level, step, _p(ids))
+        return nel, nact, ids
+
+    def min_rec(self):
+        return lib().orc_min_rec(self.h)
+
+    def kernel(self, xs, y, src_idx=0):
+        out = np.zeros(2)
+        lib().orc_kernel_eval(self.h, _p(np.ascontiguousarray(xs, np.float64)),
+                              _p(np.ascontiguousarray(y, np.float64)), src_idx, _p(out))
+        return complex(out[0], out[1])
+
+    def direct(self, w, idx, nthreads=None):
+        """FP64 direct sum at targets[idx] over all sources (self term 0)."""
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+        out = np.zeros(idx.size, np.complex128)
+        n = self._normals
+        lib().orc_direct(self.h, _p(self._srcs), self._srcs.shape[0], _p(w),
+                         None if n is None else _p(n), _p(self._tgts), _p(idx), idx.size,
+                         _p(out), nthreads or default_threads())
+        return out
+
+
+def chebnodes1d(n):
+    out = np.zeros(n)
+    lib().orc_chebnodes1d(n, _p(out))
+    return out
+
+
+def chebtransform(ns, values):
+    ns_a = np.asarray(ns, dtype=np.int32)
+    v = np.ascontiguousarray(values, dtype=np.complex128)
+    out = np.zeros_like(v)
+    lib().orc_chebtransform(_p(ns_a), _p(v), _p(out))
+    return out
+
+
+def clenshaw(ns, coef, pts):
+    ns_a = np.asarray(ns, dtype=np.int32)
+    c = np.ascontiguousarray(coef, dtype=np.complex128)
+    p = _pts(pts)
+    out = np.zeros(p.shape[0], np.complex128)
+    lib().orc_clenshaw(_p(ns_a), _p(c), _p(p), p.shape[0], _p(out))
+    return out
+
+
+def morton_sort(pts):
+    p = _pts(pts)
+    perm = np.zeros(p.shape[0], np.int64)
+    box = np.zeros(6)
+    lib().orc_morton_sort(_p(p), p.shape[0], _p(perm), _p(box))
+    return perm, box
+
+
+def morton_less(box, p, q):
+    return bool(lib().orc_morton_less(_p(np.ascontiguousarray(box, np.float64)),
+                                      _p(np.ascontiguousarray(p, np.float64)),
+                                      _p(np.ascontiguousarray(q, np.float64))))
+
+
+def cart_to_interp(x, xc, H):
+    out = np.zeros(3)
+    lib().orc_cart_to_interp(_p(np.ascontiguousarray(x, np.float64)),
+                             _p(np.ascontiguousarray(xc, np.float64)), float(H), _p(out))
+    return out
+
+
+def interp_to_cart(p, xc, H):
+    out = np.zeros(3)
+    lib().orc_interp_to_cart(_p(np.ascontiguousarray(p, np.float64)),
+                             _p(np.ascontiguousarray(xc, np.float64)), float(H), _p(out))
+    return out
