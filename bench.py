@@ -1,0 +1,230 @@
+"""Benchmark: IFGF matvecs/s of the Helmholtz single-layer operator on one MI355X.
+
+Workload (the headline configuration): N = 1,048,576 points uniformly random on the unit
+sphere, targets = sources, kappa = 100, Chebyshev order 6, leaf size 64, complex N(0,1)
+densities.  A step is one mult (one matvec) with the densities already resident in HBM.
+The mixed form (FP32 arithmetic, FP64 cone coordinates / phases / geometry) is the measured
+value; the plain FP64 form is timed beside it, and the relative error of both against an
+FP64 direct sum on sampled targets is reported.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def sphere(n, seed):
+    g = np.random.default_rng(seed).standard_normal((n, 3))
+    return g / np.linalg.norm(g, axis=1)[:, None]
+
+
+def direct_sum_torch(x, w, idx, kappa, dev):
+    """FP64 direct sum sum_i w_i e^{i kappa d}/(4 pi d), d != 0, at targets x[idx] (torch)."""
+    import torch
+    X = torch.as_tensor(x, dtype=torch.float64, device=dev)
+    W = torch.as_tensor(w, dtype=torch.complex128, device=dev)
+    out = []
+    for chunk in np.array_split(np.asarray(idx), max(1, len(idx) // 32)):
+        Y = X[torch.as_tensor(chunk, device=dev)]
+        diff = Y[:, None, :] - X[None, :, :]  # explicit differences: exact zero self-distance
+        d = torch.sqrt((diff * diff).sum(dim=2))
+        g = torch.where(d > 0, torch.exp(1j * kappa * d) / (4 * np.pi * torch.clamp(d, min=1e-300)),
+                        torch.zeros((), dtype=torch.complex128, device=dev))
+        out.append((g * W[None, :]).sum(dim=1))
+    return torch.cat(out).cpu().numpy()
+
+
+def flops_per_mult(work):
+    """Algorithmic flops of one mult (transcendentals count 1 each)."""
+    n_lo = work["n_lo"]
+    return {
+        "near": 23.0 * work["near_pairs"],
+        "leaf": 26.0 * work["leaf_pairs"],
+        "upward": (54.0 + 4.0 * n_lo) * work["up_evals"],
+        "transform": work["transform_flops_per_cone"] * work["coarse_cones"],
+        "far": (44.0 + 4.0 * n_lo) * work["far_pairs"],
+    }
+
+
+def time_mults(op, w_dev, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        op.mult(w_dev)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        u = op.mult(w_dev)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps, u
+
+
+def cpu_baseline(x, w, kappa, order, leaf, stride, nthreads):
+    """FP64 CPU restatement of the reference (the oracle) on every stride-th level-2 subtree."""
+    from oracle import pyoracle as po
+    orc = po.Oracle(po.HELMHOLTZ, kappa, leaf, order, 1, -1)
+    t0 = time.time()
+    orc.init(x, x, nthreads=nthreads)
+    t_init = time.time() - t0
+    t0 = time.time()
+    frac = orc.mult_sample(w, stride, nthreads=nthreads)
+    t = time.time() - t0
+    full = t / max(frac, 1e-12)
+    return {
+        "value": 1.0 / full,
+        "unit": "matvecs/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": (f"recursive phase of every {stride}th subtree at the first level with >32 "
+                   f"boxes ({frac * 100:.1f}% of the source points) timed {t:.1f}s and scaled "
+                   f"by the point fraction; plan build {t_init:.1f}s untimed"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--kappa", type=float, default=100.0)
+    ap.add_argument("--order", type=int, default=6)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--samples", type=int, default=1000)
+    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
+              "shards nothing across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ifgf_amd
+
+    dev = torch.device("cuda:0")
+    N = args.n
+    x = sphere(N, 1002)
+    rng = np.random.default_rng(2002)
+    w = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    idx = np.random.default_rng(3002).choice(N, size=min(args.samples, N), replace=False)
+    print(f"# N={N} kappa={args.kappa} order={args.order} leaf={args.leaf}", file=sys.stderr,
+          flush=True)
+
+    t0 = time.time()
+    op = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
+                                        precision="mixed")
+    op.init(x.T, x.T)
+    t_init = time.time() - t0
+    work = op.plan_work()
+    print(f"# init {t_init:.1f}s  {op.plan_counts()}", file=sys.stderr, flush=True)
+    w_dev = torch.as_tensor(w, dtype=torch.complex128, device=dev)
+    ms, u = time_mults(op, w_dev, args.steps, args.warmup)
+    print(f"# mixed: {ms:.3f} ms/matvec", file=sys.stderr, flush=True)
+
+    # per-phase device times (instrumented pass, outside the timed region)
+    os.environ["IFGF_PHASE_TIMING"] = "1"
+    op.mult(w_dev)
+    torch.cuda.synchronize()
+    phases = op.last_timings()
+    del os.environ["IFGF_PHASE_TIMING"]
+    print(f"# phases (ms): {phases}", file=sys.stderr, flush=True)
+
+    ref = direct_sum_torch(x, w, idx, args.kappa, dev)
+    u_np = u.cpu().numpy()
+    err = float(np.linalg.norm(u_np[idx] - ref) / np.linalg.norm(ref))
+
+    fp64 = None
+    if not args.no_fp64:
+        op64 = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
+                                              precision="fp64")
+        op64.init(x.T, x.T)
+        ms64, u64 = time_mults(op64, w_dev, max(2, args.steps // 3), 1)
+        u64_np = u64.cpu().numpy()
+        fp64 = {
+            "ms_per_step": ms64,
+            "value": 1000.0 / ms64,
+            "rel_err_vs_direct": float(np.linalg.norm(u64_np[idx] - ref) / np.linalg.norm(ref)),
+            "mixed_vs_fp64_rel_diff": float(np.linalg.norm(u_np - u64_np) / np.linalg.norm(u64_np)),
+        }
+        print(f"# fp64: {ms64:.3f} ms/matvec", file=sys.stderr, flush=True)
+        del op64
+        torch.cuda.empty_cache()
+
+    fl = flops_per_mult(work)
+    names = {"near": "near", "leaf": "leaf", "upward": "upward", "transform": "transform",
+             "far": "far"}
+    dom = max(names, key=lambda k: phases.get(k, 0.0))
+    t_dom = phases[dom] * 1e-3
+    achieved = fl[dom] / t_dom / 1e12 if t_dom > 0 else 0.0
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": None,
+        "kernel": dom,
+        "note": "compute-bound on the FP32 vector ALUs (MI355X FP32 vector peak = FP32 matrix "
+                "peak = 157.3 TFLOP/s); algorithmic flops per mult / the phase's device time",
+        "phase_ms": phases,
+        "phase_tflops": {k: (fl[k] / (phases[k] * 1e-3) / 1e12 if phases.get(k, 0) > 0 else None)
+                         for k in fl},
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        nthreads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(x, w, args.kappa, args.order, args.leaf, args.cpu_stride, nthreads)
+
+    line = {
+        "metric": "IFGF matvecs/sec at N=1M, kappa=100 (Helmholtz single layer, mixed FP32 form)",
+        "value": 1000.0 / ms,
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 cone coordinates, phases and geometry)",
+        "data": "synthetic",
+        "config": {
+            "workload": "helmholtz_single_layer_sphere",
+            "n_points": N,
+            "kappa": args.kappa,
+            "order": args.order,
+            "leaf_size": args.leaf,
+            "targets": "= sources",
+            "densities": "complex N(0,1), seed 2002; points seed 1002",
+            "parallelism": "single GPU",
+        },
+        "rel_err_vs_direct": err,
+        "rel_err_samples": int(len(idx)),
+        "init_s": t_init,
+        "fp64_form": fp64,
+        "mixed_over_fp64": (fp64["ms_per_step"] / ms) if fp64 else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

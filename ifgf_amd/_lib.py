@@ -39,6 +39,7 @@ SIGNATURES = {
     "ifgf_plan_tree": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ifgf_plan_pairs": (_I, [_P, _I, _P, _P]),
     "ifgf_plan_cones": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "ifgf_plan_work": (_I, [_P, _P]),
     "ifgf_last_timings": (_I, [_P, _P]),
 }
 

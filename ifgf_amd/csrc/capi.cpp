@@ -261,6 +261,54 @@ extern "C"
         return IFGF_OK;
     }
 
+    int ifgf_plan_work(const ifgf_operator* op, double* o)
+    {
+        if (!op || !o)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        const Plan& P = op->plan;
+        const Tree& S = P.S;
+        const Tree& T = P.T;
+        double near = 0, leaf = 0, up = 0, far = 0, leafc = 0;
+        const int n_hi = P.ho_order[0] * P.ho_order[1] * P.ho_order[2];
+        const int n_lo = P.lo_order[0] * P.lo_order[1] * P.lo_order[2];
+        for (const Pair& p : P.near)
+            near += double(S.end[p.src] - S.begin[p.src]) * double(T.end[p.tgt] - T.begin[p.tgt]);
+        for (const Pair& p : P.far)
+            far += double(T.end[p.tgt] - T.begin[p.tgt]);
+        for (int64_t c = 0; c < P.num_cones(); c++)
+        {
+            const int32_t b = P.cone_box[c];
+            if (S.leaf[b])
+            {
+                leafc += 1;
+                leaf += double(n_hi) * double(S.end[b] - S.begin[b]);
+            }
+            else
+            {
+                int nch = 0;
+                for (int j = 0; j < 8; j++)
+                    nch += S.child[8 * b + j] >= 0;
+                up += double(n_hi) * nch;
+            }
+        }
+        const int h0 = P.ho_order[0], h1 = P.ho_order[1], h2 = P.ho_order[2];
+        const int m0 = P.factor[0] * P.lo_order[0], m1 = P.factor[1] * P.lo_order[1],
+                  m2 = P.factor[2] * P.lo_order[2];
+        o[0] = near;
+        o[1] = leaf;
+        o[2] = up;
+        o[3] = far;
+        o[4] = double(P.num_cones());
+        o[5] = double(P.n_fine);
+        o[6] = leafc;
+        o[7] = n_hi;
+        o[8] = n_lo;
+        o[9] = 4.0 * (double(m0) * h1 * h2 * h0 + double(m0) * m1 * h2 * h1 + double(m0) * m1 * m2 * h2);
+        return IFGF_OK;
+    }
+
     int ifgf_last_timings(const ifgf_operator* op, double* ms6)
     {
         if (!op || !ms6)

@@ -176,6 +176,13 @@ class _IfgfOperator:
         return dict(nc=nc, nf=nf, cone_off=cone_off, cell=cell, mask=mask, fine_off=fine_off,
                     fine_ids=fine_ids)
 
+    def plan_work(self):
+        w = np.zeros(10)
+        check(lib().ifgf_plan_work(self._h, w.ctypes.data), self._h, "plan_work")
+        keys = ["near_pairs", "leaf_pairs", "up_evals", "far_pairs", "coarse_cones", "fine_cones",
+                "leaf_cones", "n_hi", "n_lo", "transform_flops_per_cone"]
+        return dict(zip(keys, w.tolist()))
+
     def last_timings(self):
         t = np.zeros(6)
         check(lib().ifgf_last_timings(self._h, t.ctypes.data), self._h, "last_timings")

@@ -91,6 +91,12 @@ int ifgf_plan_pairs(const ifgf_operator* op, int far, int32_t* src, int32_t* tgt
  * cone: cell id and fine sub-cone mask; fine_ids: active fine cells per box, ascending */
 int ifgf_plan_cones(const ifgf_operator* op, int64_t* nc, int64_t* nf, int64_t* cone_off,
                     int64_t* cone_cell, uint32_t* mask, int64_t* fine_off, int64_t* fine_ids);
+/* algorithmic work of one mult (per density vector), from the plan:
+ * [0] near point pairs  [1] leaf (node, source) pairs  [2] upward (node, child) evaluations
+ * [3] far (target, box) pairs  [4] coarse cones  [5] fine cones  [6] leaf coarse cones
+ * [7] n_hi  [8] n_lo  [9] transform flops per coarse cone */
+int ifgf_plan_work(const ifgf_operator* op, double* out10);
+
 /* per-phase device times of the last mult in ms (near, leaf, upward, transform, far, total);
  * filled only when the environment variable IFGF_PHASE_TIMING is set */
 int ifgf_last_timings(const ifgf_operator* op, double* ms6);

@@ -1340,6 +1340,46 @@ struct Operator
             transfer_interp(nd, fine, pn, pg, pg.active[m], pnodes, &(*up)[m * pst]);
     }
 
+    // the recursive phase restricted to every stride-th box of the first recursive level;
+    // returns the fraction of source points in the sampled subtrees (bounded CPU timing)
+    double mult_sample(const cplx* weights, int stride) const
+    {
+        std::vector<cplx> w(nsrc);
+        for (int64_t i = 0; i < nsrc; i++)
+            w[i] = weights[S.perm[i]];
+        if (min_rec >= S.levels() || min_rec == 0)
+            return 0.0;
+        const int pl = min_rec - 1;
+        std::vector<InterpData> idata(S.lv[pl].size());
+        for (size_t p = 0; p < S.lv[pl].size(); p++)
+        {
+            const Node& pn = S.lv[pl][p];
+            K.order_for_box(pn.bb.side, base_order, 1, idata[p].order);
+            idata[p].grid = &pn.grid[1];
+            idata[p].values.assign(pn.grid[1].active.size() * idata[p].stride(), cplx(0));
+        }
+        const auto& Lr = S.lv[min_rec];
+        std::vector<int64_t> ids;
+        int64_t pts = 0;
+        for (size_t i = 0; i < Lr.size(); i += size_t(stride))
+        {
+            ids.push_back(int64_t(i));
+            pts += Lr[i].end - Lr[i].begin;
+        }
+        std::vector<std::vector<cplx>> thread_res(std::max(1, nthreads));
+        for (auto& r : thread_res)
+            r.assign(ntgt, cplx(0));
+        parallel_for(int64_t(ids.size()), nthreads,
+                     [&](int64_t q, int th)
+                     {
+                         const Node& nd = Lr[ids[q]];
+                         std::vector<cplx> up(idata[nd.parent].values.size(), cplx(0));
+                         recursive_mult(min_rec, int(ids[q]), w.data(), thread_res[th].data(),
+                                        &idata[nd.parent], &up);
+                     });
+        return double(pts) / double(nsrc);
+    }
+
     // ifgfoperator.hpp:246-867
     std::vector<cplx> mult(const cplx* weights) const
     {
@@ -1569,6 +1609,24 @@ extern "C"
             return -1;
         }
         return 0;
+    }
+
+    // cutOff callback of init (ifgfoperator.hpp:65-67) as a distance threshold: admissible
+    // pairs farther than dmax are dropped (dmax <= 0: keep all)
+    void orc_set_cutoff(void* hv, double dmax)
+    {
+        auto* h = static_cast<OrcHandle*>(hv);
+        if (dmax > 0)
+            h->op->cutoff = [dmax](double d) { return d > dmax; };
+        else
+            h->op->cutoff = nullptr;
+    }
+
+    double orc_mult_sample(void* hv, const double* w, int stride, int nthreads)
+    {
+        auto* h = static_cast<OrcHandle*>(hv);
+        h->op->nthreads = nthreads;
+        return h->op->mult_sample(reinterpret_cast<const cplx*>(w), stride);
     }
 
     void orc_base_order(void* hv, int* out)

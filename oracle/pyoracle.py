@@ -37,6 +37,9 @@ def lib():
         L.orc_init.argtypes = [P, P, I64, P, I64, P, I]
         L.orc_mult.argtypes = [P, P, P, I]
         L.orc_base_order.argtypes = [P, P]
+        L.orc_set_cutoff.argtypes = [P, D]
+        L.orc_mult_sample.restype = D
+        L.orc_mult_sample.argtypes = [P, P, I, I]
         L.orc_levels.argtypes = [P, I]
         L.orc_num_boxes.restype = I64
         L.orc_num_boxes.argtypes = [P, I, I]
@@ -88,9 +91,17 @@ class Oracle:
                                   float(tol), int(dx))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
-            self.h = None
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+    def set_cutoff(self, dmax):
+        """Drop admissible pairs with box distance > dmax at the next init (<= 0: keep all)."""
+        lib().orc_set_cutoff(self.h, float(dmax))
+        return self
 
     def init(self, srcs, tgts, normals=None, nthreads=None):
         s = _pts(srcs)
@@ -111,6 +122,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error(self.h).decode())
         return out
+
+    def mult_sample(self, w, stride, nthreads=None):
+        """Recursive phase on every stride-th first-recursive-level subtree; returns the
+        fraction of source points covered (for a bounded CPU timing)."""
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
+        return lib().orc_mult_sample(self.h, _p(w), int(stride), nthreads or default_threads())
 
     def base_order(self):
         o = np.zeros(3, dtype=np.int32)

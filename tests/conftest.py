@@ -1,0 +1,47 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and libifgf_amd.so")
+
+
+def sphere(n, seed):
+    g = np.random.default_rng(seed).standard_normal((n, 3))
+    return g / np.linalg.norm(g, axis=1)[:, None]
+
+
+def spheroid(n, seed, c=0.5):
+    """Points on the spheroid (1,1,c), area-uniform by rejection."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while sum(len(o) for o in out) < n:
+        g = rng.standard_normal((2 * n, 3))
+        p = g / np.linalg.norm(g, axis=1)[:, None]
+        q = p * np.array([1.0, 1.0, c])
+        # surface element of the map from the unit sphere
+        dens = np.sqrt((c * p[:, 0]) ** 2 + (c * p[:, 1]) ** 2 + p[:, 2] ** 2)
+        keep = rng.random(2 * n) < dens / dens.max()
+        out.append(q[keep])
+    return np.concatenate(out)[:n]
+
+
+def cdens(n, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(n) + 1j * r.standard_normal(n)
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Fails (does not skip) when the library cannot see a HIP device."""
+    import ifgf_amd._lib as L
+    n = L.lib().ifgf_device_count()
+    assert n > 0, "no HIP device visible to libifgf_amd.so"
+    return n

@@ -1,0 +1,158 @@
+"""HIP mult against the oracle on the same seeded inputs (FP64 form: to rounding; mixed FP32
+form: to the stated tolerance), for every kernel family and the reference's edge cases."""
+import numpy as np
+import pytest
+
+import ifgf_amd
+from conftest import cdens, sphere, spheroid
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+# relative l2 difference to the oracle allowed per precision: the FP64 form differs from the
+# oracle only by summation order; the mixed form carries FP32 arithmetic (~1e-7 per step)
+TOL = {"fp64": 1e-11, "mixed": 2e-5}
+
+CLS = {
+    po.HELMHOLTZ: ifgf_amd.HelmholtzIfgfOperator,
+    po.MODHELMHOLTZ: ifgf_amd.MofifiedHelmholtzIfgfOperator,
+    po.DOUBLELAYER: ifgf_amd.DoubleLayerHelmholtzIfgfOperator,
+    po.COMBINED: ifgf_amd.CombinedFieldHelmholtzIfgfOperator,
+    po.GRAD: ifgf_amd.GradHelmholtzIfgfOperator,
+}
+
+
+def run_pair(kind, k, leaf, order, n_elem, tol, x, y, w, prec, normals=None, dx=-1,
+             cutoff=None):
+    print(f"case kind={kind} k={k} leaf={leaf} order={order} n_elem={n_elem} tol={tol} "
+          f"N={len(x)} M={len(y)} prec={prec} dx={dx}", flush=True)
+    orc = po.Oracle(kind, k, leaf, order, n_elem, tol, dx=dx)
+    if cutoff:
+        orc.set_cutoff(cutoff)
+    orc.init(x, y, normals=normals)
+    ref = orc.mult(w)
+    if kind == po.LAPLACE:
+        op = ifgf_amd.LaplaceIfgfOperator(leaf, order, n_elem, tol, precision=prec)
+    else:
+        op = CLS[kind](k, leaf, order, n_elem, tol, precision=prec)
+    if kind == po.GRAD:
+        op.setDx(dx)
+    cb = (lambda d: d > cutoff) if cutoff else None
+    if normals is not None:
+        op.init(x.T, y.T, normals.T, cutOff=cb)
+    else:
+        op.init(x.T, y.T, cutOff=cb)
+    u = op.mult(w.real if kind == po.LAPLACE else w)
+    if kind == po.LAPLACE:
+        ref = ref.real
+    return u, ref, orc, op
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind,k,leaf,order,n_elem,tol,N", [
+    (po.HELMHOLTZ, 8.0, 64, 6, 1, -1, 6000),
+    (po.HELMHOLTZ, 20.0, 32, 8, 1, -1, 8000),
+    (po.HELMHOLTZ, 10.0, 64, 6, 1, 1e-6, 10000),       # configuration 1 (tolerance path)
+    (po.HELMHOLTZ, 12.0, 64, 6, 2, -1, 6000),
+    (po.MODHELMHOLTZ, 2.35 + 43.6j, 300, 8, 1, -1, 20000),  # test/test_ifgf.cpp:27-28,72
+    (po.MODHELMHOLTZ, 0.5 - 9.0j, 64, 8, 1, -1, 6000),
+    (po.LAPLACE, 0.0, 64, 7, 1, -1, 6000),
+])
+def test_sphere_matches_oracle(gpu, prec, kind, k, leaf, order, n_elem, tol, N):
+    x = sphere(N, 100 + N % 97)
+    w = cdens(N, 200)
+    u, ref, _, _ = run_pair(kind, k, leaf, order, n_elem, tol, x, x, w, prec)
+    assert rel(u, ref) < TOL[prec], rel(u, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_normals_kernels_match_oracle(gpu, prec):
+    x = spheroid(7000, 31)
+    y = sphere(5000, 32) * 1.3
+    n = x * np.array([1.0, 1.0, 4.0])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    w = cdens(len(x), 33)
+    for kind, k in ((po.DOUBLELAYER, 0.5 + 9.0j), (po.COMBINED, 9.0)):
+        u, ref, _, _ = run_pair(kind, k, 40, 8, 1, -1, x, y, w, prec, normals=n)
+        assert rel(u, ref) < TOL[prec] * 5, (kind, rel(u, ref))
+        # targets = sources as in a boundary integral operator
+        u, ref, _, _ = run_pair(kind, k, 40, 8, 1, -1, x, x, w, prec, normals=n)
+        assert rel(u, ref) < TOL[prec] * 5, (kind, rel(u, ref))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("dx", [-1, 0, 1, 2])
+def test_grad_matches_oracle(gpu, prec, dx):
+    x = sphere(6000, 34)
+    w = cdens(6000, 35)
+    u, ref, _, _ = run_pair(po.GRAD, 1.0 + 12.0j, 64, 6, 1, -1, x, x, w, prec, dx=dx)
+    assert rel(u, ref) < TOL[prec] * 5, rel(u, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_edge_cases_match_oracle(gpu, prec):
+    rng = np.random.default_rng(36)
+    # duplicates and boundary points in a volume cloud (many near pairs, zero distances)
+    x = rng.uniform(-1, 1, (4000, 3))
+    x[:400] = np.round(x[:400] * 4) / 4
+    x[400:500] = x[500:600]
+    w = cdens(4000, 37)
+    u, ref, _, _ = run_pair(po.HELMHOLTZ, 6.0, 16, 6, 1, -1, x, x, w, prec)
+    assert rel(u, ref) < TOL[prec]
+    # fewer points than a leaf (near field only) and a single point
+    for n in (40, 1):
+        x = sphere(n, 38)
+        w = cdens(n, 39)
+        u, ref, _, _ = run_pair(po.HELMHOLTZ, 6.0, 64, 6, 1, -1, x, x, w, prec)
+        assert np.allclose(u, ref, rtol=1e-6, atol=1e-12)
+    # targets elsewhere than the sources, and no level with more than 32 boxes
+    x = sphere(3000, 40)
+    y = sphere(2000, 41) * 2.5
+    w = cdens(3000, 42)
+    u, ref, _, _ = run_pair(po.HELMHOLTZ, 7.0, 128, 6, 1, -1, x, y, w, prec)
+    assert rel(u, ref) < TOL[prec]
+    # cutOff drops far pairs beyond a distance (ifgfoperator.hpp:65-67)
+    x = sphere(6000, 43)
+    w = cdens(6000, 44)
+    u, ref, _, op = run_pair(po.HELMHOLTZ, 8.0, 32, 6, 1, -1, x, x, w, prec, cutoff=1.2)
+    assert op.plan_counts()["dropped"] > 0
+    assert rel(u, ref) < TOL[prec]
+
+
+def test_multiple_rhs_and_device_tensors(gpu):
+    import torch
+    x = sphere(5000, 45)
+    W = np.stack([cdens(5000, 46 + i) for i in range(3)], axis=1)
+    op = ifgf_amd.HelmholtzIfgfOperator(9.0, 64, 6, 1, -1)
+    op.init(x.T, x.T)
+    U = op.mult(W)
+    assert U.shape == (5000, 3)
+    for i in range(3):
+        assert np.allclose(U[:, i], op.mult(W[:, i]), rtol=0, atol=0)
+    Ut = op.mult(torch.as_tensor(W, device="cuda:0"))
+    torch.cuda.synchronize()
+    assert np.array_equal(Ut.cpu().numpy(), U)
+    lap = ifgf_amd.LaplaceIfgfOperator(64, 7, 1, -1)
+    lap.init(x.T, x.T)
+    r = lap.mult(W[:, 0].real)
+    assert r.dtype == np.float64 and r.shape == (5000,)
+
+
+def test_configuration1_error_vs_direct_sum(gpu):
+    """N=10,000 sphere, kappa=10, leaf 64, order 6, tol 1e-6; 1,000 sampled targets."""
+    x = sphere(10000, 1001)
+    w = cdens(10000, 2001)
+    idx = np.random.default_rng(3001).choice(10000, 1000, replace=False)
+    orc = po.Oracle(po.HELMHOLTZ, 10.0, 64, 6, 1, 1e-6).init(x, x)
+    d = orc.direct(w, idx)
+    for prec in ("fp64", "mixed"):
+        op = ifgf_amd.HelmholtzIfgfOperator(10.0, 64, 6, 1, 1e-6, precision=prec)
+        op.init(x.T, x.T)
+        u = op.mult(w)
+        err = rel(u[idx], d)
+        print(f"configuration 1 {prec}: rel. error vs direct sum {err:.3e}", flush=True)
+        assert err < 5e-5
