@@ -6,13 +6,18 @@
 //   high-order-values -> fine-coefficients transform -> far-field evaluation per target
 //   level -> scatter.
 // Every kernel is templated on the arithmetic type R (double: the plain form; float: the
-// mixed form, which keeps cone coordinates, phases and box geometry in FP64).
+// mixed form, which keeps cone coordinates, phases and box geometry in FP64).  The kernels
+// that evaluate or build fine interpolants are specialised on the interpolation orders of
+// the common configurations (orders 6 and 8), with a runtime-order fallback.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <stdexcept>
+#include <utility>
 
 #include "device.hpp"
 
@@ -57,8 +62,8 @@ struct KParams
     int n_hi, n_lo;
 };
 
-constexpr double PI_D = 3.14159265358979323846;
 constexpr double TWO_PI_D = 6.28318530717958647692;
+constexpr double INV_2PI_D = 0.15915494309189533577;
 constexpr double INV_4PI_D = 0.07957747154594766788;
 
 enum
@@ -71,15 +76,19 @@ enum
     KG = 5  // gradient
 };
 
-// e^{-k delta} for the arithmetic type R; delta in FP64 is reduced before the FP32 sincos
+template <typename R> __device__ __forceinline__ void sincos_r(R ph, R* s, R* c)
+{
+    if constexpr (sizeof(R) == 4)
+        sincosf(ph, s, c);
+    else
+        sincos(ph, s, c);
+}
+
+// e^{-k delta} for a moderate argument (|k delta| of at most a few hundred radians)
 template <typename R> __device__ __forceinline__ Cx<R> expmk(double kr, double ki, R delta)
 {
-    R ph = R(-ki) * delta;
     R s, c;
-    if constexpr (sizeof(R) == 4)
-        sincosf(ph, &s, &c);
-    else
-        sincos(ph, &s, &c);
+    sincos_r<R>(R(-ki) * delta, &s, &c);
     if (kr != 0.0)
     {
         R m = exp(R(-kr) * delta);
@@ -88,16 +97,16 @@ template <typename R> __device__ __forceinline__ Cx<R> expmk(double kr, double k
     return {c, s};
 }
 
-// e^{-k r} with a phase that can be large (hundreds of radians): reduce in FP64
+// e^{-k r} with r in FP64; in the mixed form the phase is reduced modulo 2 pi in FP64
+// before the FP32 sincos
 template <typename R> __device__ __forceinline__ Cx<R> expmk_far(double kr, double ki, double r)
 {
-    double ph = -ki * r;
+    const double ph = -ki * r;
     R s, c;
     if constexpr (sizeof(R) == 4)
     {
-        double n = rint(ph * (1.0 / TWO_PI_D));
-        float red = float(fma(-n, TWO_PI_D, ph));
-        sincosf(red, &s, &c);
+        const double n = rint(ph * INV_2PI_D);
+        sincosf(float(fma(-n, TWO_PI_D, ph)), &s, &c);
     }
     else
         sincos(ph, &s, &c);
@@ -110,9 +119,10 @@ template <typename R> __device__ __forceinline__ Cx<R> expmk_far(double kr, doub
 }
 
 // ------------------------------------------------------------------------------------
-// cone location: physical offset v (target - box centre, FP64) -> fine cone slot and local
+// cone location: physical offset v (point - box centre, FP64) -> fine cone slot and local
 // coordinates in [-1,1]^3 (util.hpp:156-171 cartToInterp2, cone_domain.hpp:173-184,232-256).
-// The steps are IEEE FP64 without contraction so the cells agree with the host plan.
+// The cell search multiplies by 1/h; within 1e-9 of a cell boundary it repeats the exact
+// division the plan used, so the chosen cell is the plan's cell.
 // ------------------------------------------------------------------------------------
 struct Loc
 {
@@ -133,29 +143,21 @@ __device__ __forceinline__ bool locate(const BoxGeo& B, const int32_t* __restric
     const double r = sqrt(a + v2 * v2);
     p[0] = B.H / r;
     out.r = r;
-    double d2 = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-    {
-        if (B.lo[k] > p[k])
-        {
-            double t = B.lo[k] - p[k];
-            d2 += t * t;
-        }
-        else if (p[k] > B.hi[k])
-        {
-            double t = p[k] - B.hi[k];
-            d2 += t * t;
-        }
-    }
-    if (d2 > 0)
+    // inside the closed domain box (theta and phi always are)
+    if (p[0] < B.lo[0] || p[0] > B.hi[0])
         return false;
     int q[3];
 #pragma unroll
     for (int j = 0; j < 3; j++)
     {
-        int t = int(floor((p[j] - B.lo[j]) / B.hf[j]));
-        q[j] = min(max(t, 0), B.nf[j] - 1);
+        const double d = p[j] - B.lo[j];
+        const double t = d * B.inv_hf[j];
+        double fl = floor(t);
+        const double fr = t - fl;
+        if (fr < 1e-9 || fr > 1.0 - 1e-9)
+            fl = floor(d / B.hf[j]);
+        q[j] = min(max(int(fl), 0), B.nf[j] - 1);
+        out.xi[j] = 2.0 * (t - double(q[j])) - 1.0;
     }
     const int c0 = q[0] / f0, c1 = q[1] / f1, c2 = q[2] / f2;
     const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
@@ -167,21 +169,94 @@ __device__ __forceinline__ bool locate(const BoxGeo& B, const int32_t* __restric
     if (!((cn.mask >> sub) & 1u))
         return false;
     out.fine = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-    {
-        double aa = 0.5 * B.hf[j];
-        double bb = B.lo[j] + (double(q[j]) + 0.5) * B.hf[j];
-        out.xi[j] = (p[j] - bb) / aa;
-    }
     return true;
 }
 
+// physical high-order node q of coarse cone `cell` of box B (cone_domain.hpp:145-158,
+// util.hpp:96-116) with the plan's operation order; the cosines and sines of the node angles
+// come from tables computed on the host from the same arguments.  absolute: y = centre + ...
+// (as the plan forms it), otherwise y - centre.
+struct NodeTabs
+{
+    const double* ho_xi; // n_hi x 3 reference nodes
+    const double* th;    // (cos, sin) per (theta cell, theta node)
+    const double* ph;    // (cos, sin) per (phi cell, phi node)
+};
+
+__device__ __forceinline__ void cone_node(const BoxGeo& B, int cell, int q, int h0, int h1, int h2,
+                                          const NodeTabs& T, bool absolute, double y[3],
+                                          double& r)
+{
+#pragma clang fp contract(off)
+    const int c0 = cell % B.nc[0];
+    const int c12 = cell / B.nc[0];
+    const int c1 = c12 % B.nc[1], c2 = c12 / B.nc[1];
+    const int j12 = q / h0;
+    const int j1 = j12 % h1, j2 = j12 / h1;
+    const double a = 0.5 * B.hc[0];
+    const double b = B.lo[0] + (double(c0) + 0.5) * B.hc[0];
+    const double s = (T.ho_xi[3 * q] * a) + b;
+    r = B.H / s;
+    const int it = 2 * (B.th_off + c1 * h1 + j1), ip = 2 * (B.ph_off + c2 * h2 + j2);
+    const double ct = T.th[it], st = T.th[it + 1];
+    const double cp = T.ph[ip], sp = T.ph[ip + 1];
+    if (absolute)
+    {
+        y[0] = B.ctr[0] + r * cp * st;
+        y[1] = B.ctr[1] + r * sp * st;
+        y[2] = B.ctr[2] + r * ct;
+    }
+    else
+    {
+        y[0] = r * cp * st;
+        y[1] = r * sp * st;
+        y[2] = r * ct;
+    }
+}
+
 // ------------------------------------------------------------------------------------
-// tensor Clenshaw over a coefficient block c[i0 + o0 (i1 + o1 i2)] (chebinterp.hpp:252-346)
+// tensor Clenshaw (chebinterp.hpp:252-346).  Compile-time orders: coefficients in registers;
+// order 0: runtime fallback reading the block from memory.
 // ------------------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ Cx<R> clen_line(const Cx<R>* __restrict__ c, int n, R x)
+template <int N, typename R> __device__ __forceinline__ Cx<R> clen1c(const Cx<R>* c, R x)
+{
+    if constexpr (N == 1)
+        return c[0];
+    else if constexpr (N == 2)
+        return {c[0].x + x * c[1].x, c[0].y + x * c[1].y};
+    else
+    {
+        const R x2 = x + x;
+        Cx<R> b2 = c[N - 1];
+        Cx<R> b1 = {x2 * b2.x + c[N - 2].x, x2 * b2.y + c[N - 2].y};
+#pragma unroll
+        for (int j = N - 3; j > 0; j--)
+        {
+            Cx<R> t = {c[j].x + x2 * b1.x - b2.x, c[j].y + x2 * b1.y - b2.y};
+            b2 = b1;
+            b1 = t;
+        }
+        return {x * b1.x - b2.x + c[0].x, x * b1.y - b2.y + c[0].y};
+    }
+}
+
+template <int O0, int O1, int O2, typename R>
+__device__ __forceinline__ Cx<R> clen3c(const Cx<R>* c, R x0, R x1, R x2)
+{
+    Cx<R> plane[O2];
+#pragma unroll
+    for (int i2 = 0; i2 < O2; i2++)
+    {
+        Cx<R> line[O1];
+#pragma unroll
+        for (int i1 = 0; i1 < O1; i1++)
+            line[i1] = clen1c<O0, R>(c + O0 * (i1 + O1 * i2), x0);
+        plane[i2] = clen1c<O1, R>(line, x1);
+    }
+    return clen1c<O2, R>(plane, x2);
+}
+
+template <typename R> __device__ Cx<R> clen_line(const Cx<R>* c, int n, R x)
 {
     if (n == 1)
         return c[0];
@@ -199,9 +274,7 @@ __device__ __forceinline__ Cx<R> clen_line(const Cx<R>* __restrict__ c, int n, R
     return {x * b1.x - b2.x + c[0].x, x * b1.y - b2.y + c[0].y};
 }
 
-template <typename R>
-__device__ __forceinline__ Cx<R> clen_plane(const Cx<R>* __restrict__ c, int o0, int o1, R x0,
-                                            R x1)
+template <typename R> __device__ Cx<R> clen_plane(const Cx<R>* c, int o0, int o1, R x0, R x1)
 {
     if (o1 == 1)
         return clen_line(c, o0, x0);
@@ -225,9 +298,7 @@ __device__ __forceinline__ Cx<R> clen_plane(const Cx<R>* __restrict__ c, int o0,
     return {x1 * b1.x - b2.x + c0.x, x1 * b1.y - b2.y + c0.y};
 }
 
-template <typename R>
-__device__ __forceinline__ Cx<R> clen3(const Cx<R>* __restrict__ c, int o0, int o1, int o2, R x0,
-                                       R x1, R x2v)
+template <typename R> __device__ Cx<R> clen3r(const Cx<R>* c, int o0, int o1, int o2, R x0, R x1, R x2v)
 {
     const int s = o0 * o1;
     if (o2 == 1)
@@ -252,6 +323,40 @@ __device__ __forceinline__ Cx<R> clen3(const Cx<R>* __restrict__ c, int o0, int 
     return {x2v * b1.x - b2.x + c0.x, x2v * b1.y - b2.y + c0.y};
 }
 
+// value of the fine interpolant in slot `slot` at local coordinates xi
+template <int O0, int O1, int O2, typename R>
+__device__ __forceinline__ Cx<R> fine_eval(const Cx<R>* __restrict__ F, int slot, const KParams& kp,
+                                           const double* xi)
+{
+    if constexpr (O0 > 0)
+    {
+        constexpr int N = O0 * O1 * O2;
+        Cx<R> c[N];
+        const Cx<R>* p = F + int64_t(slot) * N;
+        if constexpr (sizeof(R) == 4 && (N % 2) == 0)
+        {
+            const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+            for (int i = 0; i < N / 2; i++)
+            {
+                const float4 v = p4[i];
+                c[2 * i] = {v.x, v.y};
+                c[2 * i + 1] = {v.z, v.w};
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < N; i++)
+                c[i] = p[i];
+        }
+        return clen3c<O0, O1, O2, R>(c, R(xi[0]), R(xi[1]), R(xi[2]));
+    }
+    else
+        return clen3r<R>(F + int64_t(slot) * kp.n_lo, kp.lo0, kp.lo1, kp.lo2, R(xi[0]), R(xi[1]),
+                         R(xi[2]));
+}
+
 // ------------------------------------------------------------------------------------
 // kernel functions.  P2P: G(x_src - y) (helmholtz_ifgf.hpp:37-41 and siblings)
 // ------------------------------------------------------------------------------------
@@ -270,7 +375,6 @@ __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const 
             if (d < R(1e-12))
                 return {0, 0};
             const R xd = kp.dx == 0 ? x0 : (kp.dx == 1 ? x1 : x2);
-            // -(1/4pi) (1/d^2) e^{-kd} (-k - 1/d) x_dx
             Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
             Cx<R> mk = {R(-kp.k_re) - R(1) / d, R(-kp.k_im)};
             Cx<R> v = cmul(e, mk);
@@ -305,10 +409,7 @@ __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const 
         const R nxy = -(R(nrm[0]) * x0 + R(nrm[1]) * x1 + R(nrm[2]) * x2);
         const R kap = R(kp.kappa);
         R s, c;
-        if constexpr (sizeof(R) == 4)
-            sincosf(kap * dd, &s, &c);
-        else
-            sincos(kap * dd, &s, &c);
+        sincos_r<R>(kap * dd, &s, &c);
         const R f = inv4pi * invd * invd * invd;
         const R re = f * (c * nxy + s * kap * (dd * nxy + d2));
         const R im = f * (-c * kap * (dd * nxy + d2) + s * nxy);
@@ -373,31 +474,15 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
         const R nxy = -(z0 * nrm[0] + z1 * nrm[1] + z2 * nrm[2]);
         const R kap = R(kp.kappa);
         R s, c;
-        if constexpr (sizeof(R) == 4)
-            sincosf(kap * delta, &s, &c);
-        else
-            sincos(kap * delta, &s, &c);
+        sincos_r<R>(kap * delta, &s, &c);
         const R f = dc / (d2 * d);
         const R a = w.x * c - w.y * s, b = w.x * s + w.y * c;
         return {f * (a * nxy + b * kap * (d * nxy + d2)), f * (-a * kap * (d * nxy + d2) + b * nxy)};
     }
 }
 
-// centre factor CF(y - xc) at distance r (helmholtz_ifgf.hpp:43-65 single-point branch)
-template <int KIND, typename R> __device__ __forceinline__ Cx<R> centre_factor(const KParams& kp, double r)
-{
-    const R a = R(INV_4PI_D / r);
-    if constexpr (KIND == KL)
-        return {a, 0};
-    else
-    {
-        Cx<R> e = expmk_far<R>(kp.k_re, kp.k_im, r);
-        return cscale(e, a);
-    }
-}
-
 // ------------------------------------------------------------------------------------
-// kernels
+// simple kernels
 // ------------------------------------------------------------------------------------
 template <typename R>
 __global__ void k_gather(const double* __restrict__ w_in, const int64_t* __restrict__ perm,
@@ -437,11 +522,6 @@ __device__ __forceinline__ int pow2_at_least(int n)
     while (p < n)
         p <<= 1;
     return p;
-}
-
-template <typename R> __device__ __forceinline__ R wave_xor(R v, int m)
-{
-    return __shfl_xor(v, m, 64);
 }
 
 // near field: for each target of the item, sum over the points of its near source leaves
@@ -484,36 +564,11 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
     }
     for (int m = Tp; m < 64; m <<= 1)
     {
-        acc.x += wave_xor(acc.x, m);
-        acc.y += wave_xor(acc.y, m);
+        acc.x += __shfl_xor(acc.x, m, 64);
+        acc.y += __shfl_xor(acc.y, m, 64);
     }
     if (grp == 0 && valid)
         result[t] = cadd(result[t], acc);
-}
-
-// node of coarse cone `cell` of box B for reference node xi (cone_domain.hpp:145-158,
-// util.hpp:96-116): returns y - centre and r = H / s
-__device__ __forceinline__ void cone_node(const BoxGeo& B, int cell, const double* xi, double y[3],
-                                          double& r)
-{
-#pragma clang fp contract(off)
-    int idx[3];
-    idx[0] = cell % B.nc[0];
-    cell /= B.nc[0];
-    idx[1] = cell % B.nc[1];
-    idx[2] = cell / B.nc[1];
-    double p[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-    {
-        const double a = 0.5 * B.hc[i];
-        const double b = B.lo[i] + (double(idx[i]) + 0.5) * B.hc[i];
-        p[i] = (xi[i] * a) + b;
-    }
-    r = B.H / p[0];
-    y[0] = r * cos(p[2]) * sin(p[1]);
-    y[1] = r * sin(p[2]) * sin(p[1]);
-    y[2] = r * cos(p[1]);
 }
 
 // leaf sampling: factored kernel of the leaf's sources at the high-order nodes of one
@@ -526,8 +581,7 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
                                               const int64_t* __restrict__ send,
                                               const double* __restrict__ spts,
                                               const double* __restrict__ snrm,
-                                              const Cx<R>* __restrict__ w,
-                                              const double* __restrict__ ho_nodes,
+                                              const Cx<R>* __restrict__ w, NodeTabs tabs,
                                               Cx<R>* __restrict__ V, KParams kp)
 {
     constexpr int CH = 128;
@@ -547,7 +601,7 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
         const bool active = q < kp.n_hi;
         double yd[3] = {0, 0, 0}, r = 1;
         if (active)
-            cone_node(B, cn.cell, ho_nodes + 3 * q, yd, r);
+            cone_node(B, cn.cell, q, kp.ho0, kp.ho1, kp.ho2, tabs, false, yd, r);
         const R y0 = R(yd[0]), y1 = R(yd[1]), y2 = R(yd[2]), dc = R(r);
         Cx<R> acc = {0, 0};
         for (int64_t c0 = b0; c0 < b1; c0 += CH)
@@ -588,15 +642,14 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
     }
 }
 
-// upward pass: children's fine interpolants at the parent's high-order nodes times the
-// transfer factor, summed over children in order (ifgfoperator.hpp:631-688, 1639-1705)
-template <int KIND, typename R>
+// upward pass: the children's fine interpolants at the parent's high-order nodes times the
+// transfer factor, summed over the children in order (ifgfoperator.hpp:631-688, 1639-1705)
+template <bool PH, typename R, int O0, int O1, int O2>
 __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, int64_t nwork,
                                             const ConeDev* __restrict__ cones,
                                             const BoxGeo* __restrict__ geo,
                                             const int32_t* __restrict__ child,
-                                            const int32_t* __restrict__ table,
-                                            const double* __restrict__ ho_nodes,
+                                            const int32_t* __restrict__ table, NodeTabs tabs,
                                             const Cx<R>* __restrict__ F, Cx<R>* __restrict__ V,
                                             KParams kp)
 {
@@ -605,11 +658,11 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
         return;
     const int32_t g = work[wi];
     const ConeDev cn = cones[g];
-    const BoxGeo P = geo[cn.box];
+    const BoxGeo& P = geo[cn.box];
     for (int q = threadIdx.x; q < kp.n_hi; q += blockDim.x)
     {
         double y[3], rp;
-        cone_node(P, cn.cell, ho_nodes + 3 * q, y, rp);
+        cone_node(P, cn.cell, q, kp.ho0, kp.ho1, kp.ho2, tabs, true, y, rp);
         Cx<R> acc = {0, 0};
         for (int c = 0; c < 8; c++)
         {
@@ -618,16 +671,13 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
                 continue;
             const BoxGeo& C = geo[ch];
             Loc L;
-            const double v0 = y[0] - (C.ctr[0] - P.ctr[0]);
-            const double v1 = y[1] - (C.ctr[1] - P.ctr[1]);
-            const double v2 = y[2] - (C.ctr[2] - P.ctr[2]);
-            if (!locate(C, table, cones, v0, v1, v2, kp.f0, kp.f1, kp.f2, L))
+            if (!locate(C, table, cones, y[0] - C.ctr[0], y[1] - C.ctr[1], y[2] - C.ctr[2],
+                        kp.f0, kp.f1, kp.f2, L))
                 continue;
-            Cx<R> val = clen3<R>(F + int64_t(L.fine) * kp.n_lo, kp.lo0, kp.lo1, kp.lo2,
-                                 R(L.xi[0]), R(L.xi[1]), R(L.xi[2]));
+            const Cx<R> val = fine_eval<O0, O1, O2, R>(F, L.fine, kp, L.xi);
             // e^{-k (d - dp)} dp / d with d = |y - x_child|, dp = |y - x_parent|
             Cx<R> tf;
-            if constexpr (KIND == KL)
+            if constexpr (!PH)
                 tf = {R(rp / L.r), R(0)};
             else
                 tf = cscale(expmk<R>(kp.k_re, kp.k_im, R(L.r - rp)), R(rp / L.r));
@@ -639,7 +689,134 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
 
 // fused high-order values -> fine coefficients of the active sub-cones, separable per axis:
 // D = (W2 x W1 x W0) V with W_d the (f o_lo) x o_hi axis maps (chebinterp.hpp:176-250,
-// ifgfoperator.hpp:1539-1637 and the fine transform :548-570)
+// ifgfoperator.hpp:1539-1637 and the fine transform :548-570).  One thread per pencil of an
+// axis; the active fine cones are written contiguously at the end.
+template <typename R, int H0, int H1, int H2, int L0, int L1, int L2, int F0, int F1, int F2,
+          int CPB>
+__global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
+                                                     const ConeDev* __restrict__ cones,
+                                                     const Cx<R>* __restrict__ V,
+                                                     Cx<R>* __restrict__ Fout,
+                                                     const R* __restrict__ W)
+{
+    constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
+    constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2, NSUB = F0 * F1 * F2;
+    constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
+    constexpr int YS = (M0 * H1 * H2 > M0 * M1 * M2) ? M0 * H1 * H2 : M0 * M1 * M2;
+    constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
+    __shared__ R Ws[NW0 + NW1 + NW2];
+    __shared__ Cx<R> X[CPB * XS];
+    __shared__ Cx<R> Y[CPB * YS];
+    __shared__ unsigned char subl[CPB][32];
+    __shared__ int nsub[CPB];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NW0 + NW1 + NW2; i += 256)
+        Ws[i] = W[i];
+    const int64_t first = c0 + int64_t(blockIdx.x) * CPB;
+    const int ncone = int(min<int64_t>(CPB, c1 - first));
+    for (int e = tid; e < ncone * NH; e += 256)
+    {
+        const int k = e / NH, i = e - k * NH;
+        X[k * XS + i] = V[(first + k) * NH + i];
+    }
+    if (tid < ncone)
+    {
+        const uint32_t mask = cones[first + tid].mask;
+        int n = 0;
+        for (int s = 0; s < NSUB; s++)
+            if ((mask >> s) & 1u)
+                subl[tid][n++] = (unsigned char)s;
+        nsub[tid] = n;
+    }
+    __syncthreads();
+    const R* W0 = Ws;
+    const R* W1 = Ws + NW0;
+    const R* W2 = Ws + NW0 + NW1;
+    // axis 0: pencils (i1, i2): Y[j0 + M0 (i1 + H1 i2)] = sum_i0 W0[j0][i0] X[i0 + H0 (i1 + H1 i2)]
+    for (int e = tid; e < ncone * H1 * H2; e += 256)
+    {
+        const int k = e / (H1 * H2), p = e - k * (H1 * H2);
+        Cx<R> in[H0];
+#pragma unroll
+        for (int t = 0; t < H0; t++)
+            in[t] = X[k * XS + p * H0 + t];
+#pragma unroll
+        for (int j = 0; j < M0; j++)
+        {
+            Cx<R> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H0; t++)
+            {
+                a.x += W0[j * H0 + t] * in[t].x;
+                a.y += W0[j * H0 + t] * in[t].y;
+            }
+            Y[k * YS + p * M0 + j] = a;
+        }
+    }
+    __syncthreads();
+    // axis 1: pencils (j0, i2): X[j0 + M0 (j1 + M1 i2)] = sum_i1 W1[j1][i1] Y[j0 + M0 (i1 + H1 i2)]
+    for (int e = tid; e < ncone * M0 * H2; e += 256)
+    {
+        const int k = e / (M0 * H2), p = e - k * (M0 * H2);
+        const int j0 = p % M0, i2 = p / M0;
+        Cx<R> in[H1];
+#pragma unroll
+        for (int t = 0; t < H1; t++)
+            in[t] = Y[k * YS + j0 + M0 * (t + H1 * i2)];
+#pragma unroll
+        for (int j = 0; j < M1; j++)
+        {
+            Cx<R> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H1; t++)
+            {
+                a.x += W1[j * H1 + t] * in[t].x;
+                a.y += W1[j * H1 + t] * in[t].y;
+            }
+            X[k * XS + j0 + M0 * (j + M1 * i2)] = a;
+        }
+    }
+    __syncthreads();
+    // axis 2: pencils (j0, j1): Y[j0 + M0 (j1 + M1 j2)] = sum_i2 W2[j2][i2] X[j0 + M0 (j1 + M1 i2)]
+    for (int e = tid; e < ncone * M0 * M1; e += 256)
+    {
+        const int k = e / (M0 * M1), p = e - k * (M0 * M1);
+        Cx<R> in[H2];
+#pragma unroll
+        for (int t = 0; t < H2; t++)
+            in[t] = X[k * XS + p + M0 * M1 * t];
+#pragma unroll
+        for (int j = 0; j < M2; j++)
+        {
+            Cx<R> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H2; t++)
+            {
+                a.x += W2[j * H2 + t] * in[t].x;
+                a.y += W2[j * H2 + t] * in[t].y;
+            }
+            Y[k * YS + p + M0 * M1 * j] = a;
+        }
+    }
+    __syncthreads();
+    // write the active sub-cones' coefficient blocks contiguously
+    for (int k = 0; k < ncone; k++)
+    {
+        const int n = nsub[k];
+        const int base = cones[first + k].fine_base;
+        for (int e = tid; e < n * NLO; e += 256)
+        {
+            const int rk = e / NLO, coef = e - rk * NLO;
+            const int sub = subl[k][rk];
+            const int s0 = sub % F0, s1 = (sub / F0) % F1, s2 = sub / (F0 * F1);
+            const int m0 = coef % L0, m1 = (coef / L0) % L1, m2 = coef / (L0 * L1);
+            const int j0 = s0 * L0 + m0, j1 = s1 * L1 + m1, j2 = s2 * L2 + m2;
+            Fout[int64_t(base + rk) * NLO + coef] = Y[k * YS + j0 + M0 * (j1 + M1 * j2)];
+        }
+    }
+}
+
+// runtime-order version of the transform (any orders and factors)
 template <typename R>
 __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int cpb, int xsz,
                                                    int ysz, const ConeDev* __restrict__ cones,
@@ -669,7 +846,6 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
     const R* W0 = Ws;
     const R* W1 = Ws + w1off;
     const R* W2 = Ws + w2off;
-    // axis 0: Y[j0 + m0 (i1 + h1 i2)] = sum_i0 W0[j0][i0] X[i0 + h0 (i1 + h1 i2)]
     {
         const int n = m0 * h1 * h2;
         for (int e = threadIdx.x; e < ncone * n; e += blockDim.x)
@@ -688,7 +864,6 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
         }
     }
     __syncthreads();
-    // axis 1: X[j0 + m0 (j1 + m1 i2)] = sum_i1 W1[j1][i1] Y[j0 + m0 (i1 + h1 i2)]
     {
         const int n = m0 * m1 * h2;
         for (int e = threadIdx.x; e < ncone * n; e += blockDim.x)
@@ -707,10 +882,9 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
         }
     }
     __syncthreads();
-    // axis 2 and scatter to the active fine sub-cones
     {
         const int n = m0 * m1 * m2;
-        const int lo0 = kp.lo0, lo1 = kp.lo1, lo2 = kp.lo2;
+        const int lo0 = kp.lo0, lo1 = kp.lo1;
         for (int e = threadIdx.x; e < ncone * n; e += blockDim.x)
         {
             const int k = e / n, i = e - k * n;
@@ -724,13 +898,14 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
                 a.y += wv * x[m0 * m1 * t].y;
             }
             const int j0 = j01 % m0, j1 = j01 / m0;
-            const int s0 = j0 / lo0, s1 = j1 / lo1, s2 = j2 / lo2;
+            const int s0 = j0 / lo0, s1 = j1 / lo1, s2 = j2 / kp.lo2;
             const int sub = s0 + kp.f0 * (s1 + kp.f1 * s2);
             const ConeDev cn = cones[first + k];
             if ((cn.mask >> sub) & 1u)
             {
                 const int slot = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
-                const int coef = (j0 - s0 * lo0) + lo0 * ((j1 - s1 * lo1) + lo1 * (j2 - s2 * lo2));
+                const int coef =
+                    (j0 - s0 * lo0) + lo0 * ((j1 - s1 * lo1) + lo1 * (j2 - s2 * kp.lo2));
                 F[int64_t(slot) * kp.n_lo + coef] = a;
             }
         }
@@ -739,7 +914,7 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
 
 // far field: for each target of the item, the interpolants of its far source boxes times the
 // centre factor (ifgfoperator.hpp:962-1012, 1708-1891)
-template <int KIND, typename R>
+template <bool PH, typename R, int O0, int O1, int O2>
 __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, int64_t nitems,
                                             const int32_t* __restrict__ far_off,
                                             const int32_t* __restrict__ far_src,
@@ -763,22 +938,34 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
     const double y0 = tpts[3 * t], y1 = tpts[3 * t + 1], y2 = tpts[3 * t + 2];
     Cx<R> acc = {0, 0};
     const int32_t j0 = far_off[wi.box], j1 = far_off[wi.box + 1];
-    for (int32_t j = j0 + grp; j < j1; j += ngrp)
+    auto body = [&](const BoxGeo& B)
     {
-        const int32_t s = far_src[j];
-        const BoxGeo& B = geo[s];
         Loc L;
         if (!locate(B, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0, kp.f1,
                     kp.f2, L))
-            continue;
-        Cx<R> val = clen3<R>(F + int64_t(L.fine) * kp.n_lo, kp.lo0, kp.lo1, kp.lo2, R(L.xi[0]),
-                             R(L.xi[1]), R(L.xi[2]));
-        acc = cadd(acc, cmul(val, centre_factor<KIND, R>(kp, L.r)));
-    }
-    for (int m = Tp; m < 64; m <<= 1)
+            return;
+        const Cx<R> val = fine_eval<O0, O1, O2, R>(F, L.fine, kp, L.xi);
+        Cx<R> cf;
+        if constexpr (!PH)
+            cf = {R(INV_4PI_D / L.r), R(0)};
+        else
+            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, L.r), R(INV_4PI_D / L.r));
+        acc = cadd(acc, cmul(val, cf));
+    };
+    if (ngrp == 1)
     {
-        acc.x += wave_xor(acc.x, m);
-        acc.y += wave_xor(acc.y, m);
+        for (int32_t j = j0; j < j1; j++) // uniform source box: scalar loads of its geometry
+            body(geo[__builtin_amdgcn_readfirstlane(far_src[j])]);
+    }
+    else
+    {
+        for (int32_t j = j0 + grp; j < j1; j += ngrp)
+            body(geo[far_src[j]]);
+        for (int m = Tp; m < 64; m <<= 1)
+        {
+            acc.x += __shfl_xor(acc.x, m, 64);
+            acc.y += __shfl_xor(acc.y, m, 64);
+        }
     }
     if (grp == 0 && valid)
         result[t] = cadd(result[t], acc);
@@ -844,15 +1031,94 @@ struct Timing
 };
 Timing g_last;
 
+// which compile-time order set matches the plan: 1 = (3,6,6)/(2,3,3)/3, 2 = (5,8,8)/(2,5,5)/3
+int order_class(const DevicePlan& D)
+{
+    const bool f3 = D.f[0] == 3 && D.f[1] == 3 && D.f[2] == 3;
+    if (f3 && D.ho[0] == 3 && D.ho[1] == 6 && D.ho[2] == 6 && D.lo[0] == 2 && D.lo[1] == 3 &&
+        D.lo[2] == 3)
+        return 1;
+    if (f3 && D.ho[0] == 5 && D.ho[1] == 8 && D.ho[2] == 8 && D.lo[0] == 2 && D.lo[1] == 5 &&
+        D.lo[2] == 5)
+        return 2;
+    return 0;
+}
+
+template <bool PH, typename R, int O0, int O1, int O2>
+void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs,
+                     bool up, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
+{
+    if (up)
+        k_up<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
+            D.up_cones + a, b - a, D.cones, D.geo, D.child, D.table, tabs, F, V, kp);
+    else
+        k_far<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
+            D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.table, D.cones, F, D.tgt_pts,
+            res, kp);
+}
+
+template <bool PH, typename R>
+void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs, bool up,
+                   int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
+{
+    switch (order_class(D))
+    {
+    case 1:
+        return launch_up_far_t<PH, R, 2, 3, 3>(D, st, kp, tabs, up, a, b, F, V, res);
+    case 2:
+        return launch_up_far_t<PH, R, 2, 5, 5>(D, st, kp, tabs, up, a, b, F, V, res);
+    default:
+        return launch_up_far_t<PH, R, 0, 0, 0>(D, st, kp, tabs, up, a, b, F, V, res);
+    }
+}
+
+template <typename R>
+void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t c0, int64_t c1,
+                      Cx<R>* V, Cx<R>* F)
+{
+    const R* W = static_cast<const R*>(D.wmat);
+    const int cls = order_class(D);
+    if (cls == 1)
+    {
+        constexpr int CPB = sizeof(R) == 4 ? 6 : 3;
+        k_transform_t<R, 3, 6, 6, 2, 3, 3, 3, 3, 3, CPB>
+            <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V,
+                                                                             F, W);
+        return;
+    }
+    if (cls == 2)
+    {
+        constexpr int CPB = sizeof(R) == 4 ? 2 : 1;
+        k_transform_t<R, 5, 8, 8, 2, 5, 5, 3, 3, 3, CPB>
+            <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V,
+                                                                             F, W);
+        return;
+    }
+    const int h0 = D.ho[0], h1 = D.ho[1], h2 = D.ho[2];
+    const int m0 = D.f[0] * D.lo[0], m1 = D.f[1] * D.lo[1], m2 = D.f[2] * D.lo[2];
+    const int xsz = std::max(h0 * h1 * h2, m0 * m1 * h2);
+    const int ysz = std::max(m0 * h1 * h2, m0 * m1 * m2);
+    const int nW = m0 * h0 + m1 * h1 + m2 * h2;
+    const size_t per_cone = size_t(xsz + ysz) * sizeof(Cx<R>);
+    const int cpb = int(std::max<size_t>(1, std::min<size_t>(8, (48 * 1024) / per_cone)));
+    const size_t smem = ((nW + 3) & ~3) * sizeof(R) + cpb * per_cone;
+    if (smem > 64 * 1024)
+        throw std::runtime_error("interpolation orders too large for the transform kernel's LDS");
+    k_transform<R><<<dim3(unsigned((c1 - c0 + cpb - 1) / cpb)), dim3(256), smem, st>>>(
+        c0, c1, cpb, xsz, ysz, D.cones, V, F, W, D.woff[1], D.woff[2], kp);
+}
+
 template <int KIND, typename R>
 void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed)
 {
+    constexpr bool PH = KIND != KL;
     const KParams kp = kparams(D);
+    const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
     Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
-    hipEvent_t ev[7];
+    hipEvent_t ev[6];
     if (timed)
         for (auto& e : ev)
             HIP_OK(hipEventCreate(&e));
@@ -873,21 +1139,15 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     if (D.n_leaf_cones)
         k_leaf<KIND, R><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(
             D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
-            D.normals, ws, D.ho_nodes, V, kp);
+            D.normals, ws, tabs, V, kp);
     mark(2);
-    // separable transform launch shape
-    const int h0 = D.ho[0], h1 = D.ho[1], h2 = D.ho[2];
-    const int m0 = D.f[0] * D.lo[0], m1 = D.f[1] * D.lo[1], m2 = D.f[2] * D.lo[2];
-    const int xsz = std::max(h0 * h1 * h2, m0 * m1 * h2);
-    const int ysz = std::max(m0 * h1 * h2, m0 * m1 * m2);
-    const int nW = m0 * h0 + m1 * h1 + m2 * h2;
-    const size_t per_cone = size_t(xsz + ysz) * sizeof(Cx<R>);
-    const int cpb = int(std::max<size_t>(1, std::min<size_t>(8, (48 * 1024) / per_cone)));
-    const size_t smem = ((nW + 3) & ~3) * sizeof(R) + cpb * per_cone;
     float t_up = 0, t_tr = 0;
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
+        const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
+        if (u1 == u0 && c1 == c0)
+            continue;
         hipEvent_t a = nullptr, b = nullptr, c = nullptr;
         if (timed)
         {
@@ -897,18 +1157,11 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
             HIP_OK(hipEventRecord(a, st));
         }
         if (u1 > u0)
-            k_up<KIND, R><<<dim3(unsigned(u1 - u0)), dim3(128), 0, st>>>(
-                D.up_cones + u0, u1 - u0, D.cones, D.geo, D.child, D.table, D.ho_nodes, F, V, kp);
+            launch_up_far<PH, R>(D, st, kp, tabs, true, u0, u1, F, V, res);
         if (timed)
             HIP_OK(hipEventRecord(b, st));
-        const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
         if (c1 > c0)
-        {
-            const unsigned nb = unsigned((c1 - c0 + cpb - 1) / cpb);
-            k_transform<R><<<dim3(nb), dim3(256), smem, st>>>(
-                c0, c1, cpb, xsz, ysz, D.cones, V, F, static_cast<const R*>(D.wmat), D.woff[1],
-                D.woff[2], kp);
-        }
+            launch_transform<R>(D, st, kp, c0, c1, V, F);
         if (timed)
         {
             HIP_OK(hipEventRecord(c, st));
@@ -928,9 +1181,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     {
         const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
         if (b > a)
-            k_far<KIND, R><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
-                D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.table, D.cones, F,
-                D.tgt_pts, res, kp);
+            launch_up_far<PH, R>(D, st, kp, tabs, false, a, b, F, V, res);
     }
     mark(4);
     k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.tgt_perm,
@@ -953,7 +1204,6 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
         g_last.ms[5] = x;
         for (int i = 0; i < 6; i++)
             (void)hipEventDestroy(ev[i]);
-        (void)hipEventDestroy(ev[6]);
     }
 }
 
@@ -990,9 +1240,10 @@ bool device_available()
 void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
-                    D.src_end, D.child, D.geo, D.cones, D.table, D.ho_nodes, D.wmat,
-                    D.leaf_cones, D.up_cones, D.far_off, D.far_src, D.near_off, D.near_src,
-                    D.far_items, D.near_items, D.wsorted, D.result, D.V, D.F};
+                    D.src_end, D.child, D.geo, D.cones, D.table, D.ho_nodes, D.trig_th,
+                    D.trig_ph, D.wmat, D.leaf_cones, D.up_cones, D.far_off, D.far_src,
+                    D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
+                    D.F};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1039,6 +1290,11 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         D.src_begin = dupload(P.S.begin, tot);
         D.src_end = dupload(P.S.end, tot);
         D.child = dupload(P.S.child, tot);
+        // node-angle tables per coarse-grid class (theta cells x nodes, phi cells x nodes), with
+        // the plan's arguments (Grid::point) so node positions are bit-identical to the plan's
+        const auto xth = chebnodes1d(D.ho[1]), xph = chebnodes1d(D.ho[2]);
+        std::vector<double> th, ph;
+        std::map<std::pair<int64_t, int64_t>, std::pair<int32_t, int32_t>> cls;
         const int64_t nb = P.S.num_boxes();
         std::vector<BoxGeo> geo(nb);
         for (int64_t g = 0; g < nb; g++)
@@ -1050,14 +1306,44 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                 B.lo[d] = P.dom_lo[3 * g + d];
                 B.hi[d] = P.dom_hi[3 * g + d];
                 B.hf[d] = P.hf[3 * g + d];
+                B.inv_hf[d] = 1.0 / P.hf[3 * g + d];
                 B.hc[d] = P.hc[3 * g + d];
                 B.nf[d] = int32_t(P.nf[3 * g + d]);
                 B.nc[d] = int32_t(P.nc[3 * g + d]);
             }
             B.H = P.S.side[g];
             B.table_off = P.table_off[g];
+            const auto key = std::make_pair(P.nc[3 * g + 1], P.nc[3 * g + 2]);
+            auto it = cls.find(key);
+            if (it == cls.end())
+            {
+                const int32_t to = int32_t(th.size() / 2), po = int32_t(ph.size() / 2);
+                for (int64_t c = 0; c < P.nc[3 * g + 1]; c++)
+                    for (int j = 0; j < D.ho[1]; j++)
+                    {
+                        const double a = 0.5 * B.hc[1];
+                        const double b = B.lo[1] + (double(c) + 0.5) * B.hc[1];
+                        const double v = (xth[j] * a) + b;
+                        th.push_back(std::cos(v));
+                        th.push_back(std::sin(v));
+                    }
+                for (int64_t c = 0; c < P.nc[3 * g + 2]; c++)
+                    for (int j = 0; j < D.ho[2]; j++)
+                    {
+                        const double a = 0.5 * B.hc[2];
+                        const double b = B.lo[2] + (double(c) + 0.5) * B.hc[2];
+                        const double v = (xph[j] * a) + b;
+                        ph.push_back(std::cos(v));
+                        ph.push_back(std::sin(v));
+                    }
+                it = cls.emplace(key, std::make_pair(to, po)).first;
+            }
+            B.th_off = it->second.first;
+            B.ph_off = it->second.second;
         }
         D.geo = dupload(geo, tot);
+        D.trig_th = dupload(th, tot);
+        D.trig_ph = dupload(ph, tot);
         std::vector<ConeDev> cones(D.n_cones);
         for (int64_t c = 0; c < D.n_cones; c++)
             cones[c] = {P.cone_box[c], int32_t(P.cone_cell[c]), P.cone_mask[c],

@@ -20,13 +20,16 @@ struct BoxGeo
 {
     double ctr[3];
     double H;
-    double lo[3];  // cone domain (s, theta, phi)
+    double lo[3];     // cone domain (s, theta, phi)
     double hi[3];
-    double hf[3];  // fine cell widths
-    double hc[3];  // coarse cell widths
-    int32_t nf[3]; // fine cells per axis
-    int32_t nc[3]; // coarse cells per axis
+    double hf[3];     // fine cell widths
+    double inv_hf[3]; // 1 / hf (cell search; exact division near cell boundaries)
+    double hc[3];     // coarse cell widths
+    int32_t nf[3];    // fine cells per axis
+    int32_t nc[3];    // coarse cells per axis
     int64_t table_off;
+    int32_t th_off;   // node-angle tables of the coarse grid (cos/sin per cell and node)
+    int32_t ph_off;
 };
 
 struct ConeDev
@@ -71,6 +74,8 @@ struct DevicePlan
     ConeDev* cones = nullptr;
     int32_t* table = nullptr;
     double* ho_nodes = nullptr;   // n_hi x 3 reference nodes in [-1,1]^3
+    double* trig_th = nullptr;    // (cos, sin) of theta at coarse cell / node, per grid class
+    double* trig_ph = nullptr;    // (cos, sin) of phi
     void* wmat = nullptr;         // axis maps (R): W0 | W1 | W2
     int woff[4] = {0, 0, 0, 0};
 
