@@ -46,10 +46,20 @@ SIGNATURES = {
 _lib = None
 
 
+def _load_torch_first():
+    """torch ships its own HIP runtime; it must be loaded before this library's runtime is
+    initialised, otherwise torch cannot initialise HIP in the same process."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load the in-tree shared object (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        _load_torch_first()
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m ifgf_amd.build` "

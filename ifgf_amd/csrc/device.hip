@@ -1096,14 +1096,19 @@ void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t 
     }
     const int h0 = D.ho[0], h1 = D.ho[1], h2 = D.ho[2];
     const int m0 = D.f[0] * D.lo[0], m1 = D.f[1] * D.lo[1], m2 = D.f[2] * D.lo[2];
+    // X holds the values and the axis-1 stage, Y the axis-0 stage; axis 2 goes to memory
     const int xsz = std::max(h0 * h1 * h2, m0 * m1 * h2);
-    const int ysz = std::max(m0 * h1 * h2, m0 * m1 * m2);
+    const int ysz = m0 * h1 * h2;
     const int nW = m0 * h0 + m1 * h1 + m2 * h2;
     const size_t per_cone = size_t(xsz + ysz) * sizeof(Cx<R>);
     const int cpb = int(std::max<size_t>(1, std::min<size_t>(8, (48 * 1024) / per_cone)));
     const size_t smem = ((nW + 3) & ~3) * sizeof(R) + cpb * per_cone;
-    if (smem > 64 * 1024)
+    constexpr size_t LDS_MAX = 160 * 1024; // gfx950: 160 KB of LDS per CU
+    if (smem > LDS_MAX)
         throw std::runtime_error("interpolation orders too large for the transform kernel's LDS");
+    if (smem > 64 * 1024)
+        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_transform<R>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
     k_transform<R><<<dim3(unsigned((c1 - c0 + cpb - 1) / cpb)), dim3(256), smem, st>>>(
         c0, c1, cpb, xsz, ysz, D.cones, V, F, W, D.woff[1], D.woff[2], kp);
 }
