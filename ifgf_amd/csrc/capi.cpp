@@ -169,6 +169,14 @@ extern "C"
         c[9] = P.n_tofar;
         c[10] = op->dev.bytes_total;
         c[11] = int64_t(P.fine_ids.size());
+        c[12] = 0;
+        c[13] = 0;
+        for (const UpLevel& U : P.up)
+            if (U.valid)
+            {
+                c[12]++;
+                c[13] += int64_t(U.entries.size());
+            }
         return IFGF_OK;
     }
 

@@ -79,6 +79,13 @@ struct DevicePlan
     void* wmat = nullptr;         // axis maps (R): W0 | W1 | W2
     int woff[4] = {0, 0, 0, 0};
 
+    // upward geometry tables (per parent level): cell -> compact index, and entries
+    // {cf, sub, xi[3], tf[2]} in R; up_tab[l] < 0 when level l uses the per-box path
+    int32_t* up_cellmap = nullptr;
+    void* up_entries = nullptr;
+    std::vector<int64_t> up_map_off, up_ent_off;
+    std::vector<int> up_tab;
+
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
     int32_t* up_cones = nullptr;             // non-leaf cones, level-major

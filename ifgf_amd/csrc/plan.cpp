@@ -721,9 +721,154 @@ int estimate_orders(Plan& P, double tol, int out[3])
     return 0;
 }
 
+// transfer factor e^{-k(d - dp)} dp / d with d = |x - xc|, dp = |x - pxc| (helmholtz_ifgf.hpp:67-92
+// and siblings; Laplace: dp / d)
+static cplx transfer_host(const KernelSpec& K, const double* x, const double* xc, const double* pxc)
+{
+    double a[3] = {x[0] - xc[0], x[1] - xc[1], x[2] - xc[2]};
+    double b[3] = {x[0] - pxc[0], x[1] - pxc[1], x[2] - pxc[2]};
+    const double d = nrm(a), dp = nrm(b);
+    if (K.kind == KIND_LAPLACE)
+        return cplx(dp / d, 0.0);
+    return std::exp(-K.k * (d - dp)) * (dp / d);
+}
+
 // ---------------------------------------------------------------------------------------
 // cone activity (octree.hpp:351-816, TwoGrid branch)
 // ---------------------------------------------------------------------------------------
+// grid of box g's coarse (step 1) or fine (step 0) cones, from the plan's per-box arrays
+static Grid box_grid(const Plan& P, int64_t g, int step)
+{
+    Grid G;
+    for (int d = 0; d < 3; d++)
+    {
+        G.n[d] = step ? P.nc[3 * g + d] : P.nf[3 * g + d];
+        G.lo[d] = P.dom_lo[3 * g + d];
+        G.hi[d] = P.dom_hi[3 * g + d];
+        G.h[d] = step ? P.hc[3 * g + d] : P.hf[3 * g + d];
+    }
+    return G;
+}
+
+static bool same_grid(const Plan& P, int64_t a, int64_t b)
+{
+    for (int d = 0; d < 3; d++)
+        if (P.nc[3 * a + d] != P.nc[3 * b + d] || P.nf[3 * a + d] != P.nf[3 * b + d] ||
+            P.hc[3 * a + d] != P.hc[3 * b + d] || P.hf[3 * a + d] != P.hf[3 * b + d] ||
+            P.dom_lo[3 * a + d] != P.dom_lo[3 * b + d] || P.dom_hi[3 * a + d] != P.dom_hi[3 * b + d])
+            return false;
+    return true;
+}
+
+// upward geometry of parent level l (children on level l + 1), see UpLevel
+static void build_up_level(Plan& P, int l, const std::vector<double>& ho_nodes, int nthreads)
+{
+    const Tree& S = P.S;
+    UpLevel& U = P.up[l];
+    U = UpLevel();
+    const int64_t p0 = S.level_off[l], p1 = S.level_off[l + 1];
+    const int64_t c0 = S.level_off[l + 1], c1 = S.level_off[l + 2];
+    // one grid class for all parents and all children of the level
+    // boxes of a level are translates up to rounding of their extents (a child's extent is
+    // (lo + size) - lo); sides must agree to 1e-12 and the grids exactly
+    auto close = [](double a, double b) { return std::abs(a - b) <= 1e-12 * std::abs(b); };
+    bool any = false;
+    for (int64_t g = p0; g < p1; g++)
+    {
+        if (!same_grid(P, g, p0) || !close(S.side[g], S.side[p0]))
+            return;
+        any |= (!S.leaf[g] && P.cone_off[g + 1] > P.cone_off[g]);
+    }
+    if (!any)
+        return;
+    for (int64_t g = c0; g < c1; g++)
+        if (!same_grid(P, g, c0) || !close(S.side[g], S.side[c0]))
+            return;
+    const size_t n_ho = ho_nodes.size() / 3;
+    const Grid pg = box_grid(P, p0, 1);
+    U.cellmap.assign(size_t(pg.total()), -1);
+    int32_t nu = 0;
+    for (int64_t g = p0; g < p1; g++)
+        if (!S.leaf[g])
+            for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
+                if (U.cellmap[P.cone_cell[c]] < 0)
+                    U.cellmap[P.cone_cell[c]] = nu++;
+    if (int64_t(nu) * int64_t(n_ho) * 8 > P.up_budget)
+    {
+        U.cellmap.clear();
+        return;
+    }
+    std::vector<int64_t> cells(nu);
+    for (int64_t c = 0; c < int64_t(U.cellmap.size()); c++)
+        if (U.cellmap[c] >= 0)
+            cells[U.cellmap[c]] = c;
+    // representative parent (the first box of the level) and its eight child boxes, formed as
+    // build_tree forms children
+    const double* pxc = &S.ctr[3 * p0];
+    const double pH = S.side[p0];
+    double cctr[8][3], cH[8];
+    for (int o = 0; o < 8; o++)
+    {
+        double lo[3], hi[3], m = 0;
+        for (int d = 0; d < 3; d++)
+        {
+            double size = 0.5 * (S.bhi[3 * p0 + d] - S.blo[3 * p0 + d]);
+            lo[d] = S.blo[3 * p0 + d] + size * double((o >> d) & 1);
+            hi[d] = lo[d] + size;
+            cctr[o][d] = (hi[d] + lo[d]) / 2.0;
+            m = std::max(m, std::abs(hi[d] - lo[d]));
+        }
+        cH[o] = m;
+        if (!close(m, S.side[c0]))
+            return; // children differ from the representative: keep the per-box path
+    }
+    const Grid fg = box_grid(P, c0, 0), cg = box_grid(P, c0, 1);
+    U.entries.assign(size_t(nu) * n_ho * 8, UpEntry());
+    const int* f = P.factor;
+    pfor(int64_t(nu) * int64_t(n_ho), nthreads,
+         [&](int64_t e, int)
+         {
+             const int64_t u = e / int64_t(n_ho);
+             const size_t q = size_t(e % int64_t(n_ho));
+             double sp[3], y[3];
+             pg.point(cells[u], &ho_nodes[3 * q], sp);
+             interp_to_cart(sp, pxc, pH, y);
+             for (int o = 0; o < 8; o++)
+             {
+                 UpEntry& E = U.entries[size_t(e) * 8 + o];
+                 double s[3];
+                 cart_to_interp(y, cctr[o], cH[o], s);
+                 const int64_t fc = fg.cell(s);
+                 if (fc < 0)
+                 {
+                     E.fine = E.coarse = E.cf = -1;
+                     E.sub = 0;
+                     continue;
+                 }
+                 E.fine = int32_t(fc);
+                 E.coarse = int32_t(cg.cell(s));
+                 int64_t t = fc, idx[3];
+                 for (int d = 0; d < 3; d++)
+                 {
+                     idx[d] = t % fg.n[d];
+                     t /= fg.n[d];
+                 }
+                 E.cf = int32_t((idx[0] / f[0]) + cg.n[0] * ((idx[1] / f[1]) + cg.n[1] * (idx[2] / f[2])));
+                 E.sub = int32_t((idx[0] % f[0]) + f[0] * ((idx[1] % f[1]) + f[1] * (idx[2] % f[2])));
+                 for (int d = 0; d < 3; d++)
+                 {
+                     const double a = 0.5 * fg.h[d];
+                     const double b = fg.lo[d] + (double(idx[d]) + 0.5) * fg.h[d];
+                     E.xi[d] = (s[d] - b) / a;
+                 }
+                 const cplx tf = transfer_host(P.K, y, cctr[o], pxc);
+                 E.tf[0] = tf.real();
+                 E.tf[1] = tf.imag();
+             }
+         });
+    U.valid = true;
+}
+
 void build_cones(Plan& P, int nthreads)
 {
     const Tree& S = P.S;
@@ -746,6 +891,7 @@ void build_cones(Plan& P, int nthreads)
     P.fine_ids.clear();
     P.n_fine = 0;
     P.n_fine_uncovered = 0;
+    P.up.assign(S.levels, UpLevel());
 
     // far pairs grouped by source box
     std::vector<int64_t> far_off(nb + 1, 0);
@@ -767,6 +913,30 @@ void build_cones(Plan& P, int nthreads)
     for (int level = 0; level < S.levels; level++)
     {
         const int64_t l0 = S.level_off[level], l1 = S.level_off[level + 1];
+        // cone grids of the level (octree.hpp:487-548)
+        for (int64_t g = l0; g < l1; g++)
+        {
+            const double H = S.side[g];
+            double smax = std::sqrt(3.0) / 3;
+            double smin = P.K.smin(H);
+            if (smin >= smax)
+                smin = smax - 0.05;
+            const double dlo[3] = {smin, 0, -PI}, dhi[3] = {smax, PI, PI};
+            int64_t nf[3], nc[3];
+            P.K.elements(H, P.base_els, 0, nf);
+            P.K.elements(H, P.base_els, 1, nc);
+            for (int d = 0; d < 3; d++)
+            {
+                P.dom_lo[3 * g + d] = dlo[d];
+                P.dom_hi[3 * g + d] = dhi[d];
+                P.nf[3 * g + d] = nf[d];
+                P.nc[3 * g + d] = nc[d];
+                P.hf[3 * g + d] = (dhi[d] - dlo[d]) / double(nf[d]);
+                P.hc[3 * g + d] = (dhi[d] - dlo[d]) / double(nc[d]);
+            }
+        }
+        if (level > 0)
+            build_up_level(P, level - 1, ho_nodes, nthreads);
         struct Out
         {
             std::vector<int64_t> cells;
@@ -781,36 +951,16 @@ void build_cones(Plan& P, int nthreads)
                  const int64_t g = l0 + q;
                  const double H = S.side[g];
                  const double* xc = &S.ctr[3 * g];
-                 double smax = std::sqrt(3.0) / 3;
-                 double smin = P.K.smin(H);
-                 if (smin >= smax)
-                     smin = smax - 0.05;
-                 Grid fg, cg;
-                 const double dlo[3] = {smin, 0, -PI}, dhi[3] = {smax, PI, PI};
-                 P.K.elements(H, P.base_els, 0, fg.n);
-                 P.K.elements(H, P.base_els, 1, cg.n);
-                 for (int d = 0; d < 3; d++)
-                 {
-                     fg.lo[d] = cg.lo[d] = dlo[d];
-                     fg.hi[d] = cg.hi[d] = dhi[d];
-                     fg.h[d] = (dhi[d] - dlo[d]) / double(fg.n[d]);
-                     cg.h[d] = (dhi[d] - dlo[d]) / double(cg.n[d]);
-                     P.dom_lo[3 * g + d] = dlo[d];
-                     P.dom_hi[3 * g + d] = dhi[d];
-                     P.nf[3 * g + d] = fg.n[d];
-                     P.nc[3 * g + d] = cg.n[d];
-                     P.hf[3 * g + d] = fg.h[d];
-                     P.hc[3 * g + d] = cg.h[d];
-                 }
+                 const Grid fg = box_grid(P, g, 0), cg = box_grid(P, g, 1);
                  std::vector<uint64_t> fbits((fg.total() + 63) / 64, 0), cbits((cg.total() + 63) / 64, 0);
-                 auto mark = [&](const double s[3])
+                 auto mark_ids = [&](int64_t c0, int64_t c1)
                  {
-                     int64_t c0 = fg.cell(s), c1 = cg.cell(s);
                      if (c1 >= 0)
                          cbits[c1 >> 6] |= uint64_t(1) << (c1 & 63);
                      if (c0 >= 0)
                          fbits[c0 >> 6] |= uint64_t(1) << (c0 & 63);
                  };
+                 auto mark = [&](const double s[3]) { mark_ids(fg.cell(s), cg.cell(s)); };
                  for (int64_t r = far_off[g]; r < far_off[g + 1]; r++)
                  {
                      const int32_t t = far_tgt[r];
@@ -824,26 +974,39 @@ void build_cones(Plan& P, int nthreads)
                  if (P.parent_has_far[g])
                  {
                      const int32_t pg = S.parent[g];
-                     const double pH = S.side[pg];
-                     const double* pxc = &S.ctr[3 * pg];
-                     Grid pgrid;
-                     for (int d = 0; d < 3; d++)
+                     const UpLevel& U = P.up[level - 1];
+                     if (U.valid)
                      {
-                         pgrid.n[d] = P.nc[3 * pg + d];
-                         pgrid.lo[d] = P.dom_lo[3 * pg + d];
-                         pgrid.hi[d] = P.dom_hi[3 * pg + d];
-                         pgrid.h[d] = P.hc[3 * pg + d];
-                     }
-                     for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
-                     {
-                         const int64_t cell = P.cone_cell[c];
-                         for (size_t k = 0; k < n_ho; k++)
+                         int oct = 0;
+                         while (S.child[8 * pg + oct] != int32_t(g))
+                             oct++;
+                         for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
                          {
-                             double sp[3], y[3], s[3];
-                             pgrid.point(cell, &ho_nodes[3 * k], sp);
-                             interp_to_cart(sp, pxc, pH, y);
-                             cart_to_interp(y, xc, H, s);
-                             mark(s);
+                             const int64_t u = U.cellmap[P.cone_cell[c]];
+                             for (size_t k = 0; k < n_ho; k++)
+                             {
+                                 const UpEntry& E = U.entries[(size_t(u) * n_ho + k) * 8 + oct];
+                                 if (E.fine >= 0)
+                                     mark_ids(E.fine, E.coarse);
+                             }
+                         }
+                     }
+                     else
+                     {
+                         const double pH = S.side[pg];
+                         const double* pxc = &S.ctr[3 * pg];
+                         const Grid pgrid = box_grid(P, pg, 1);
+                         for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
+                         {
+                             const int64_t cell = P.cone_cell[c];
+                             for (size_t k = 0; k < n_ho; k++)
+                             {
+                                 double sp[3], y[3], s[3];
+                                 pgrid.point(cell, &ho_nodes[3 * k], sp);
+                                 interp_to_cart(sp, pxc, pH, y);
+                                 cart_to_interp(y, xc, H, s);
+                                 mark(s);
+                             }
                          }
                      }
                  }
@@ -911,6 +1074,7 @@ void build_cones(Plan& P, int nthreads)
             o = Out();
         }
     }
+    // the deepest level has no children: no table
     P.table.assign(P.table_off[nb], -1);
     for (int64_t c = 0; c < P.num_cones(); c++)
         P.table[P.table_off[P.cone_box[c]] + P.cone_cell[c]] = int32_t(c);

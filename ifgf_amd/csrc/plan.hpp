@@ -77,6 +77,27 @@ struct Pair
     int32_t src, tgt; // global box ids in the source / target trees
 };
 
+// Upward geometry of one parent level.  All boxes of a level are translates of one another,
+// so where the high-order node q of parent coarse cell c lands in the fine grid of the child
+// in octant o (cell, local coordinates) and the transfer factor e^{-k(d - dp)} dp/d do not
+// depend on the box.  Entries cover the union of active coarse cells of the level's parents.
+struct UpEntry
+{
+    int32_t fine;   // fine cell of the child; -1: outside the cone domain
+    int32_t coarse; // coarse cell of the child (the reference marks both grids independently)
+    int32_t cf;     // coarse cell containing the fine cell (the cone the device reads)
+    int32_t sub;    // sub-cone index of the fine cell within cf
+    double xi[3];   // local coordinates in the fine cell
+    double tf[2];   // transfer factor (complex)
+};
+
+struct UpLevel
+{
+    bool valid = false;
+    std::vector<int32_t> cellmap;  // parent coarse cell -> compact index (or -1)
+    std::vector<UpEntry> entries;  // ((compact * n_hi) + node) * 8 + octant
+};
+
 struct Plan
 {
     KernelSpec K;
@@ -116,6 +137,11 @@ struct Plan
     // fine active cone ids per box (sorted) for parity dumps: CSR
     std::vector<int64_t> fine_off;
     std::vector<int64_t> fine_ids;
+
+    // per source level: precomputed upward geometry (valid when the level's boxes share one
+    // grid class and the table fits the budget)
+    std::vector<UpLevel> up;
+    int64_t up_budget = int64_t(64) << 20; // entries
 
     int64_t num_cones() const { return int64_t(cone_box.size()); }
 };

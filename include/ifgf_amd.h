@@ -78,7 +78,8 @@ int ifgf_device_count(void);
  * counts: [0] source levels [1] source boxes [2] target levels [3] target boxes
  *         [4] far pairs [5] near pairs [6] coarse cones [7] fine cones (covered)
  *         [8] fine cones active but not covered [9] dropped (cutOff) pairs
- *         [10] device bytes [11] fine cone ids listed */
+ *         [10] device bytes [11] fine cone ids listed
+ *         [12] levels with precomputed upward geometry [13] entries of those tables */
 int ifgf_plan_counts(const ifgf_operator* op, int64_t* counts);
 /* orders: base[3], coarse (high) order[3], fine (low) order[3], factor[3] */
 int ifgf_plan_orders(const ifgf_operator* op, int* out12);
