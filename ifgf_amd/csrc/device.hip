@@ -142,9 +142,9 @@ struct Loc
     double xi[3];
 };
 
-__device__ __forceinline__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
-                                       const ConeDev* __restrict__ cones, double v0, double v1,
-                                       double v2, int f0, int f1, int f2, Loc& out)
+__device__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
+                       const ConeDev* __restrict__ cones, double v0, double v1, double v2, int f0,
+                       int f1, int f2, Loc& out)
 {
 #pragma clang fp contract(off)
     const double a = v0 * v0 + v1 * v1;
@@ -173,12 +173,154 @@ __device__ __forceinline__ bool locate(const BoxGeo& B, const int32_t* __restric
     const int c0 = q[0] / f0, c1 = q[1] / f1, c2 = q[2] / f2;
     const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
     const int g = table[B.table_off + cell];
-    if (g < 0)
-        return false;
-    const ConeDev cn = cones[g];
     const int sub = (q[0] - c0 * f0) + f0 * ((q[1] - c1 * f1) + f1 * (q[2] - c2 * f2));
-    if (!((cn.mask >> sub) & 1u))
+    const ConeDev cn = cones[g < 0 ? 0 : g];
+    if (g < 0 || !((cn.mask >> sub) & 1u))
+    {
+        // a point on a cell edge whose cell the plan did not mark (the plan decides ties by
+        // atan2 and floor): repeat with the exact path
+        Loc L;
+        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
+            return false;
+        out.fine = L.fine;
+        out.xi[0] = float(L.xi[0]);
+        out.xi[1] = float(L.xi[1]);
+        out.xi[2] = float(L.xi[2]);
+        return true;
+    }
+    out.fine = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
+    return true;
+}
+
+// Mixed form: the same cone location without FP64 atan2.  Candidate cells come from a cheap
+// FP32 atan2 (the |err| <= 1e-5 polynomial of Abramowitz & Stegun 4.4.47); each candidate is
+// verified against its cell edges b with an FP64 cross product (theta >= b  <=>
+// rho cos b - v2 sin b >= 0; phi >= b  <=>  v1 cos b - v0 sin b >= 0) and moved by one cell if
+// needed, so the cell is decided in FP64.  The local angle coordinates come from the offset
+// rotated into the cell-centre frame: a small-angle atan in FP32, accurate to ~3e-7.
+__device__ __forceinline__ float atan2_coarse(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float z = mn * __builtin_amdgcn_rcpf(mx);
+    const float z2 = z * z;
+    float p = z * (0.9998660f +
+                   z2 * (-0.3302995f + z2 * (0.1801410f + z2 * (-0.0851330f + z2 * 0.0208351f))));
+    if (ay > ax)
+        p = 1.5707963267948966f - p;
+    if (x < 0.f)
+        p = 3.141592653589793f - p;
+    return copysignf(p, y);
+}
+
+__device__ __forceinline__ float atan_small(float u)
+{
+    if (fabsf(u) <= 0.3f)
+    {
+        const float u2 = u * u; // Taylor series to u^13: error < 1e-9 on |u| <= 0.3
+        return u * (1.f + u2 * (-1.f / 3 + u2 * (1.f / 5 + u2 * (-1.f / 7 + u2 * (1.f / 9 +
+                                                      u2 * (-1.f / 11 + u2 * (1.f / 13)))))));
+    }
+    return atanf(u);
+}
+
+struct LocF
+{
+    int fine;
+    double r;
+    float xi[3];
+};
+
+__device__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
+                       const ConeDev* __restrict__ cones, double v0, double v1, double v2, int f0,
+                       int f1, int f2, Loc& out);
+
+__device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __restrict__ ang,
+                                            const int32_t* __restrict__ table,
+                                            const ConeDev* __restrict__ cones, double v0, double v1,
+                                            double v2, int f0, int f1, int f2, LocF& out)
+{
+    double a, rho, r, s;
+    {
+#pragma clang fp contract(off)
+        a = v0 * v0 + v1 * v1;
+        rho = sqrt(a);
+        r = sqrt(a + v2 * v2);
+        s = B.H / r;
+    }
+    out.r = r;
+    if (a == 0.0) // on the polar axis: the exact path (atan2(0, 0) = 0 conventions)
+    {
+        Loc L;
+        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
+            return false;
+        out.fine = L.fine;
+        out.xi[0] = float(L.xi[0]);
+        out.xi[1] = float(L.xi[1]);
+        out.xi[2] = float(L.xi[2]);
+        return true;
+    }
+    if (s < B.lo[0] || s > B.hi[0])
         return false;
+    int q[3];
+    {
+        const double d = s - B.lo[0];
+        const double t = d * B.inv_hf[0];
+        double fl = floor(t);
+        const double fr = t - fl;
+        if (fr < 1e-9 || fr > 1.0 - 1e-9)
+            fl = floor(d / B.hf[0]);
+        q[0] = min(max(int(fl), 0), B.nf[0] - 1);
+        out.xi[0] = float(2.0 * (t - double(q[0])) - 1.0);
+    }
+    {
+        // theta in [0, pi]: edges k h1
+        const int n1 = B.nf[1];
+        int c = int(atan2_coarse(float(rho), float(v2)) * float(B.inv_hf[1]));
+        c = min(max(c, 0), n1 - 1);
+        const double* E = ang + 2 * B.fth_off;
+        if (c > 0 && !(rho * E[2 * c] - v2 * E[2 * c + 1] >= 0.0))
+            c--;
+        else if (c < n1 - 1 && rho * E[2 * c + 2] - v2 * E[2 * c + 3] >= 0.0)
+            c++;
+        q[1] = c;
+        const double* Cc = E + 2 * (n1 + 1) + 2 * c;
+        const double xr = v2 * Cc[0] + rho * Cc[1], yr = rho * Cc[0] - v2 * Cc[1];
+        out.xi[1] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[1]);
+    }
+    {
+        // phi in [-pi, pi]: edges -pi + k h2
+        const int n2 = B.nf[2];
+        int c = int((atan2_coarse(float(v1), float(v0)) + 3.141592653589793f) * float(B.inv_hf[2]));
+        c = min(max(c, 0), n2 - 1);
+        const double* E = ang + 2 * B.fph_off;
+        if (c > 0 && !(v1 * E[2 * c] - v0 * E[2 * c + 1] >= 0.0))
+            c--;
+        else if (c < n2 - 1 && v1 * E[2 * c + 2] - v0 * E[2 * c + 3] >= 0.0)
+            c++;
+        q[2] = c;
+        const double* Cc = E + 2 * (n2 + 1) + 2 * c;
+        const double xr = v0 * Cc[0] + v1 * Cc[1], yr = v1 * Cc[0] - v0 * Cc[1];
+        out.xi[2] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[2]);
+    }
+    const int c0 = q[0] / f0, c1 = q[1] / f1, c2 = q[2] / f2;
+    const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
+    const int g = table[B.table_off + cell];
+    const int sub = (q[0] - c0 * f0) + f0 * ((q[1] - c1 * f1) + f1 * (q[2] - c2 * f2));
+    const ConeDev cn = cones[g < 0 ? 0 : g];
+    if (g < 0 || !((cn.mask >> sub) & 1u))
+    {
+        // a point on a cell edge whose cell the plan did not mark (the plan decides ties by
+        // atan2 and floor): repeat with the exact path
+        Loc L;
+        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
+            return false;
+        out.fine = L.fine;
+        out.xi[0] = float(L.xi[0]);
+        out.xi[1] = float(L.xi[1]);
+        out.xi[2] = float(L.xi[2]);
+        return true;
+    }
     out.fine = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
     return true;
 }
@@ -1001,6 +1143,7 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
                                             const ConeDev* __restrict__ cones,
                                             const Cx<R>* __restrict__ F,
                                             const double* __restrict__ tpts,
+                                            const double* __restrict__ ang,
                                             Cx<R>* __restrict__ result, KParams kp)
 {
     const int64_t it = blockIdx.x;
@@ -1018,12 +1161,35 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
     const int32_t j0 = far_off[wi.box], j1 = far_off[wi.box + 1];
     auto body = [&](const BoxGeo& B)
     {
-        Loc L;
-        if (!locate(B, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0, kp.f1,
-                    kp.f2, L))
-            return;
-        const Cx<R> val =
-            fine_eval<O0, O1, O2, R>(F, L.fine, kp, R(L.xi[0]), R(L.xi[1]), R(L.xi[2]));
+        int fine;
+        double r;
+        R x0, x1, x2;
+        if constexpr (sizeof(R) == 4)
+        {
+            LocF L;
+            if (!locate_fast(B, ang, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2],
+                             kp.f0, kp.f1, kp.f2, L))
+                return;
+            fine = L.fine;
+            r = L.r;
+            x0 = L.xi[0];
+            x1 = L.xi[1];
+            x2 = L.xi[2];
+        }
+        else
+        {
+            Loc L;
+            if (!locate(B, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0,
+                        kp.f1, kp.f2, L))
+                return;
+            fine = L.fine;
+            r = L.r;
+            x0 = L.xi[0];
+            x1 = L.xi[1];
+            x2 = L.xi[2];
+        }
+        const Loc L{fine, r, {0, 0, 0}};
+        const Cx<R> val = fine_eval<O0, O1, O2, R>(F, L.fine, kp, x0, x1, x2);
         Cx<R> cf;
         if constexpr (!PH)
             cf = {R(INV_4PI_D / L.r), R(0)};
@@ -1139,7 +1305,7 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     else
         k_far<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
             D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.table, D.cones, F, D.tgt_pts,
-            res, kp);
+            D.fine_ang, res, kp);
 }
 
 template <bool PH, typename R>
@@ -1331,7 +1497,7 @@ void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
                     D.src_end, D.child, D.geo, D.cones, D.table, D.ho_nodes, D.trig_th,
-                    D.trig_ph, D.wmat, D.up_cellmap, D.up_entries, D.leaf_cones, D.up_cones, D.far_off, D.far_src,
+                    D.trig_ph, D.fine_ang, D.wmat, D.up_cellmap, D.up_entries, D.leaf_cones, D.up_cones, D.far_off, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F};
     for (void* p : ptrs)
@@ -1430,6 +1596,46 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             }
             B.th_off = it->second.first;
             B.ph_off = it->second.second;
+        }
+        {
+            // fine-grid edge and centre angles per (nf1, nf2) class: (cos, sin) of
+            // lo + k h (k = 0..n) then of lo + (k + 1/2) h (k < n), theta then phi
+            std::vector<double> fa;
+            std::map<std::pair<int64_t, int64_t>, std::pair<int32_t, int32_t>> fcls;
+            for (int64_t g = 0; g < nb; g++)
+            {
+                BoxGeo& B = geo[g];
+                const auto key = std::make_pair(P.nf[3 * g + 1], P.nf[3 * g + 2]);
+                auto it = fcls.find(key);
+                if (it == fcls.end())
+                {
+                    auto emit = [&](int axis) -> int32_t
+                    {
+                        const int32_t off = int32_t(fa.size() / 2);
+                        const int64_t n = P.nf[3 * g + axis];
+                        const double lo = B.lo[axis], h = B.hf[axis];
+                        for (int64_t k = 0; k <= n; k++)
+                        {
+                            const double v = lo + double(k) * h;
+                            fa.push_back(std::cos(v));
+                            fa.push_back(std::sin(v));
+                        }
+                        for (int64_t k = 0; k < n; k++)
+                        {
+                            const double v = lo + (double(k) + 0.5) * h;
+                            fa.push_back(std::cos(v));
+                            fa.push_back(std::sin(v));
+                        }
+                        return off;
+                    };
+                    const int32_t a1 = emit(1);
+                    const int32_t a2 = emit(2);
+                    it = fcls.emplace(key, std::make_pair(a1, a2)).first;
+                }
+                B.fth_off = it->second.first;
+                B.fph_off = it->second.second;
+            }
+            D.fine_ang = dupload(fa, tot);
         }
         D.geo = dupload(geo, tot);
         D.trig_th = dupload(th, tot);

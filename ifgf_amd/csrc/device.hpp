@@ -30,6 +30,8 @@ struct BoxGeo
     int64_t table_off;
     int32_t th_off;   // node-angle tables of the coarse grid (cos/sin per cell and node)
     int32_t ph_off;
+    int32_t fth_off;  // fine-grid theta table: (cos, sin) of the nf1+1 cell edges, then centres
+    int32_t fph_off;  // fine-grid phi table: the same for the nf2 cells
 };
 
 struct ConeDev
@@ -76,6 +78,7 @@ struct DevicePlan
     double* ho_nodes = nullptr;   // n_hi x 3 reference nodes in [-1,1]^3
     double* trig_th = nullptr;    // (cos, sin) of theta at coarse cell / node, per grid class
     double* trig_ph = nullptr;    // (cos, sin) of phi
+    double* fine_ang = nullptr;   // fine-grid edge / centre (cos, sin) tables
     void* wmat = nullptr;         // axis maps (R): W0 | W1 | W2
     int woff[4] = {0, 0, 0, 0};
 
