@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device once (a plain copy kernel of "
+                         "known size, for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
@@ -118,6 +121,12 @@ def main():
     import ifgf_amd
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration > 0:
+        a = torch.empty(args.copy_calibration << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     N = args.n
     x = sphere(N, 1002)
     rng = np.random.default_rng(2002)

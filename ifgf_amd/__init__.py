@@ -3,6 +3,7 @@
 A from-scratch HIP implementation of TimHRO/ifgf's IfgfOperator init/mult path behind the
 reference's `pyifgf` API (see ifgf_amd.pyifgf) and a C ABI (include/ifgf_amd.h).
 """
+from ._lib import IfgfError  # noqa: F401
 from .pyifgf import (  # noqa: F401
     CombinedFieldHelmholtzIfgfOperator,
     DoubleLayerHelmholtzIfgfOperator,
@@ -14,6 +15,7 @@ from .pyifgf import (  # noqa: F401
 )
 
 __all__ = [
+    "IfgfError",
     "HelmholtzIfgfOperator",
     "MofifiedHelmholtzIfgfOperator",
     "ModifiedHelmholtzIfgfOperator",

@@ -2,8 +2,10 @@
 // pipeline.  Host C++ only; the HIP code lives in device.hip.
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/ifgf_amd.h"
 #include "device.hpp"
@@ -23,6 +25,12 @@ struct ifgf_operator
     Plan plan;
     DevicePlan dev;
     std::string err;
+    // adjoint products: same_points (targets are the sources, in order) reuses this plan;
+    // otherwise a transposed operator (sources and targets swapped) is built on first use
+    bool same_points = false;
+    ifgf_cutoff_fn cutoff = nullptr;
+    void* cutoff_ctx = nullptr;
+    std::unique_ptr<ifgf_operator> adj;
 };
 
 static int fail(ifgf_operator* op, int code, const std::string& msg)
@@ -98,8 +106,13 @@ extern "C"
         try
         {
             device_free(op->dev);
+            op->adj.reset();
             op->on_device = false;
             op->planned = false;
+            op->same_points =
+                n_src == n_tgt && std::memcmp(srcs, tgts, sizeof(double) * 3 * n_src) == 0;
+            op->cutoff = cutoff;
+            op->cutoff_ctx = cutoff_ctx;
             op->plan = Plan();
             op->plan.K = op->K;
             std::function<bool(double)> cb;
@@ -140,10 +153,85 @@ extern "C"
         return IFGF_OK;
     }
 
+    int ifgf_mult_adjoint(ifgf_operator* op, const double* g, int64_t nrhs, double* out,
+                          int flags, void* stream)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!op->on_device)
+            return fail(op, IFGF_ESTATE, "mult_adjoint before a device init");
+        if (!g || !out || nrhs < 1)
+            return fail(op, IFGF_EINVAL, "mult_adjoint needs values, output and nrhs >= 1");
+        const int kind = op->K.kind;
+        if (kind == IFGF_DOUBLELAYER || kind == IFGF_COMBINED)
+            return fail(op, IFGF_EINVAL,
+                        "no adjoint for kernels with source normals (it needs target normals)");
+        // G(x - y) = G(y - x) for the potentials, and the gradient kernel is odd in x - y:
+        // A^H g = s conj(B conj(g)) with B the operator from the targets to the sources
+        const int io = IO_CONJ_IN | IO_CONJ_OUT | (kind == IFGF_GRAD ? IO_NEGATE_OUT : 0);
+        try
+        {
+            ifgf_operator* B = op;
+            if (!op->same_points)
+            {
+                if (!op->adj)
+                {
+                    auto a = std::make_unique<ifgf_operator>();
+                    a->K = op->K;
+                    a->leaf = op->leaf;
+                    a->order = op->order;
+                    a->n_elem = op->n_elem;
+                    a->precision = op->precision;
+                    a->plan.K = op->K;
+                    // the caller's point order, from the sorted points and their permutation
+                    const Tree &S = op->plan.S, &T = op->plan.T;
+                    std::vector<double> xs(3 * S.n), ys(3 * T.n);
+                    for (int64_t i = 0; i < S.n; i++)
+                        for (int d = 0; d < 3; d++)
+                            xs[3 * S.perm[i] + d] = S.pts[3 * i + d];
+                    for (int64_t i = 0; i < T.n; i++)
+                        for (int d = 0; d < 3; d++)
+                            ys[3 * T.perm[i] + d] = T.pts[3 * i + d];
+                    std::function<bool(double)> cb;
+                    if (op->cutoff)
+                    {
+                        ifgf_cutoff_fn f = op->cutoff;
+                        void* ctx = op->cutoff_ctx;
+                        cb = [f, ctx](double d) { return f(d, ctx) != 0; };
+                    }
+                    std::string e = build_plan(a->plan, ys.data(), T.n, xs.data(), S.n, nullptr,
+                                               a->leaf, a->order, a->n_elem, cb,
+                                               default_threads());
+                    if (!e.empty())
+                        return fail(op, IFGF_EINTERNAL, "adjoint plan: " + e);
+                    a->planned = true;
+                    e = device_upload(a->plan, a->precision, a->dev);
+                    if (!e.empty())
+                        return fail(op, IFGF_EINTERNAL, "adjoint upload: " + e);
+                    a->on_device = true;
+                    op->adj = std::move(a);
+                }
+                B = op->adj.get();
+                B->dev.dx = op->dev.dx;
+            }
+            std::string e = device_mult(B->dev, g, out, nrhs, (flags & IFGF_DEVICE_POINTERS) != 0,
+                                        stream, io);
+            if (!e.empty())
+                return fail(op, IFGF_EINTERNAL, e);
+        }
+        catch (const std::exception& ex)
+        {
+            return fail(op, IFGF_EINTERNAL, ex.what());
+        }
+        return IFGF_OK;
+    }
+
     void ifgf_destroy(ifgf_operator* op)
     {
         if (!op)
             return;
+        if (op->adj)
+            device_free(op->adj->dev);
         device_free(op->dev);
         delete op;
     }

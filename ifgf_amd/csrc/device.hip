@@ -63,9 +63,11 @@ struct KParams
     int n_hi, n_lo;
 };
 
+constexpr double PI_D = 3.14159265358979323846;
 constexpr double TWO_PI_D = 6.28318530717958647692;
 constexpr double INV_2PI_D = 0.15915494309189533577;
 constexpr double INV_4PI_D = 0.07957747154594766788;
+constexpr int UP_K = 8; // parent cones of one cell per upward workgroup
 
 enum
 {
@@ -142,9 +144,9 @@ struct Loc
     double xi[3];
 };
 
-__device__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
-                       const ConeDev* __restrict__ cones, double v0, double v1, double v2, int f0,
-                       int f1, int f2, Loc& out)
+__device__ __forceinline__ bool locate(const BoxGeo& B, const CellRef* __restrict__ ctab,
+                                       double v0, double v1, double v2, int f0, int f1, int f2,
+                                       Loc& out)
 {
 #pragma clang fp contract(off)
     const double a = v0 * v0 + v1 * v1;
@@ -172,23 +174,11 @@ __device__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
     }
     const int c0 = q[0] / f0, c1 = q[1] / f1, c2 = q[2] / f2;
     const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
-    const int g = table[B.table_off + cell];
+    const CellRef cr = ctab[B.table_off + cell];
     const int sub = (q[0] - c0 * f0) + f0 * ((q[1] - c1 * f1) + f1 * (q[2] - c2 * f2));
-    const ConeDev cn = cones[g < 0 ? 0 : g];
-    if (g < 0 || !((cn.mask >> sub) & 1u))
-    {
-        // a point on a cell edge whose cell the plan did not mark (the plan decides ties by
-        // atan2 and floor): repeat with the exact path
-        Loc L;
-        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
-            return false;
-        out.fine = L.fine;
-        out.xi[0] = float(L.xi[0]);
-        out.xi[1] = float(L.xi[1]);
-        out.xi[2] = float(L.xi[2]);
-        return true;
-    }
-    out.fine = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
+    if (!((cr.mask >> sub) & 1u))
+        return false;
+    out.fine = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
     return true;
 }
 
@@ -231,14 +221,10 @@ struct LocF
     float xi[3];
 };
 
-__device__ bool locate(const BoxGeo& B, const int32_t* __restrict__ table,
-                       const ConeDev* __restrict__ cones, double v0, double v1, double v2, int f0,
-                       int f1, int f2, Loc& out);
-
-__device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __restrict__ ang,
-                                            const int32_t* __restrict__ table,
-                                            const ConeDev* __restrict__ cones, double v0, double v1,
-                                            double v2, int f0, int f1, int f2, LocF& out)
+__device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* eth, const double* eph,
+                                            const CellRef* __restrict__ ctab, double v0,
+                                            double v1, double v2, int f0, int f1, int f2,
+                                            LocF& out)
 {
     double a, rho, r, s;
     {
@@ -249,17 +235,6 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __res
         s = B.H / r;
     }
     out.r = r;
-    if (a == 0.0) // on the polar axis: the exact path (atan2(0, 0) = 0 conventions)
-    {
-        Loc L;
-        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
-            return false;
-        out.fine = L.fine;
-        out.xi[0] = float(L.xi[0]);
-        out.xi[1] = float(L.xi[1]);
-        out.xi[2] = float(L.xi[2]);
-        return true;
-    }
     if (s < B.lo[0] || s > B.hi[0])
         return false;
     int q[3];
@@ -273,12 +248,12 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __res
         q[0] = min(max(int(fl), 0), B.nf[0] - 1);
         out.xi[0] = float(2.0 * (t - double(q[0])) - 1.0);
     }
+    const int n1 = B.nf[1], n2 = B.nf[2];
     {
-        // theta in [0, pi]: edges k h1
-        const int n1 = B.nf[1];
+        // theta in [0, pi]: edges k h1 (rho = 0 included: theta = 0 or pi)
         int c = int(atan2_coarse(float(rho), float(v2)) * float(B.inv_hf[1]));
         c = min(max(c, 0), n1 - 1);
-        const double* E = ang + 2 * B.fth_off;
+        const double* E = eth;
         if (c > 0 && !(rho * E[2 * c] - v2 * E[2 * c + 1] >= 0.0))
             c--;
         else if (c < n1 - 1 && rho * E[2 * c + 2] - v2 * E[2 * c + 3] >= 0.0)
@@ -288,12 +263,13 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __res
         const double xr = v2 * Cc[0] + rho * Cc[1], yr = rho * Cc[0] - v2 * Cc[1];
         out.xi[1] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[1]);
     }
+    if (a != 0.0)
     {
         // phi in [-pi, pi]: edges -pi + k h2
-        const int n2 = B.nf[2];
-        int c = int((atan2_coarse(float(v1), float(v0)) + 3.141592653589793f) * float(B.inv_hf[2]));
+        const float ph = atan2_coarse(float(v1), float(v0)) + 3.141592653589793f;
+        int c = int(ph * float(B.inv_hf[2]));
         c = min(max(c, 0), n2 - 1);
-        const double* E = ang + 2 * B.fph_off;
+        const double* E = eph;
         if (c > 0 && !(v1 * E[2 * c] - v0 * E[2 * c + 1] >= 0.0))
             c--;
         else if (c < n2 - 1 && v1 * E[2 * c + 2] - v0 * E[2 * c + 3] >= 0.0)
@@ -303,26 +279,47 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* __res
         const double xr = v0 * Cc[0] + v1 * Cc[1], yr = v1 * Cc[0] - v0 * Cc[1];
         out.xi[2] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[2]);
     }
-    const int c0 = q[0] / f0, c1 = q[1] / f1, c2 = q[2] / f2;
-    const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
-    const int g = table[B.table_off + cell];
-    const int sub = (q[0] - c0 * f0) + f0 * ((q[1] - c1 * f1) + f1 * (q[2] - c2 * f2));
-    const ConeDev cn = cones[g < 0 ? 0 : g];
-    if (g < 0 || !((cn.mask >> sub) & 1u))
+    else
     {
-        // a point on a cell edge whose cell the plan did not mark (the plan decides ties by
-        // atan2 and floor): repeat with the exact path
-        Loc L;
-        if (!locate(B, table, cones, v0, v1, v2, f0, f1, f2, L))
-            return false;
-        out.fine = L.fine;
-        out.xi[0] = float(L.xi[0]);
-        out.xi[1] = float(L.xi[1]);
-        out.xi[2] = float(L.xi[2]);
-        return true;
+        // on the polar axis phi is atan2 of signed zeros (+-0 or +-pi), placed by the same
+        // floor as the plan's
+        const double p = signbit(v0) ? (signbit(v1) ? -PI_D : PI_D) : (signbit(v1) ? -0.0 : 0.0);
+        const double d = p - B.lo[2];
+        const double t = d * B.inv_hf[2];
+        double fl = floor(t);
+        const double fr = t - fl;
+        if (fr < 1e-9 || fr > 1.0 - 1e-9)
+            fl = floor(d / B.hf[2]);
+        q[2] = min(max(int(fl), 0), n2 - 1);
+        out.xi[2] = float(2.0 * (t - double(q[2])) - 1.0);
     }
-    out.fine = cn.fine_base + __popc(cn.mask & ((1u << sub) - 1u));
-    return true;
+    const int c0 = q[0] / f0;
+    // the cell decided above, then (a point within rounding of a theta or phi edge, which the
+    // plan's atan2 and floor may have put across it) the neighbours across the nearer edges;
+    // phi wraps around
+    const int d1 = out.xi[1] > 0.f ? 1 : -1, d2 = out.xi[2] > 0.f ? 1 : -1;
+#pragma unroll 1
+    for (int m = 0; m < 4; m++)
+    {
+        const int e1 = (m & 1) ? d1 : 0, e2 = (m & 2) ? d2 : 0;
+        const int p1 = q[1] + e1;
+        int p2 = q[2] + e2;
+        p2 = p2 < 0 ? p2 + n2 : (p2 >= n2 ? p2 - n2 : p2);
+        if (p1 < 0 || p1 >= n1)
+            continue;
+        const int c1 = p1 / f1, c2 = p2 / f2;
+        const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
+        const CellRef cr = ctab[B.table_off + cell];
+        const int sub = (q[0] - c0 * f0) + f0 * ((p1 - c1 * f1) + f1 * (p2 - c2 * f2));
+        if ((cr.mask >> sub) & 1u)
+        {
+            out.fine = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
+            out.xi[1] -= float(2 * e1);
+            out.xi[2] -= float(2 * e2);
+            return true;
+        }
+    }
+    return false;
 }
 
 // physical high-order node q of coarse cone `cell` of box B (cone_domain.hpp:145-158,
@@ -451,7 +448,8 @@ template <typename R> __device__ Cx<R> clen_plane(const Cx<R>* c, int o0, int o1
     return {x1 * b1.x - b2.x + c0.x, x1 * b1.y - b2.y + c0.y};
 }
 
-template <typename R> __device__ Cx<R> clen3r(const Cx<R>* c, int o0, int o1, int o2, R x0, R x1, R x2v)
+template <typename R>
+__device__ Cx<R> clen3r(const Cx<R>* c, int o0, int o1, int o2, R x0, R x1, R x2v)
 {
     const int s = o0 * o1;
     if (o2 == 1)
@@ -629,7 +627,8 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
         sincos_r<R>(kap * delta, &s, &c);
         const R f = dc / (d2 * d);
         const R a = w.x * c - w.y * s, b = w.x * s + w.y * c;
-        return {f * (a * nxy + b * kap * (d * nxy + d2)), f * (-a * kap * (d * nxy + d2) + b * nxy)};
+        return {f * (a * nxy + b * kap * (d * nxy + d2)),
+                f * (-a * kap * (d * nxy + d2) + b * nxy)};
     }
 }
 
@@ -638,25 +637,25 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
 // ------------------------------------------------------------------------------------
 template <typename R>
 __global__ void k_gather(const double* __restrict__ w_in, const int64_t* __restrict__ perm,
-                         Cx<R>* __restrict__ w_out, int64_t n)
+                         Cx<R>* __restrict__ w_out, int64_t n, double im_sign)
 {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int64_t j = perm[i];
-    w_out[i] = {R(w_in[2 * j]), R(w_in[2 * j + 1])};
+    w_out[i] = {R(w_in[2 * j]), R(im_sign * w_in[2 * j + 1])};
 }
 
 template <typename R>
 __global__ void k_scatter(const Cx<R>* __restrict__ res, const int64_t* __restrict__ perm,
-                          double* __restrict__ out, int64_t n)
+                          double* __restrict__ out, int64_t n, double re_sign, double im_sign)
 {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int64_t j = perm[i];
-    out[2 * j] = double(res[i].x);
-    out[2 * j + 1] = double(res[i].y);
+    out[2 * j] = re_sign * double(res[i].x);
+    out[2 * j + 1] = im_sign * double(res[i].y);
 }
 
 template <typename R> __global__ void k_zero(Cx<R>* p, int64_t n)
@@ -668,6 +667,17 @@ template <typename R> __global__ void k_zero(Cx<R>* p, int64_t n)
 
 // lane layout of a 64-wide work item: Tp lanes per group (power of two >= targets), 64/Tp
 // groups splitting the interaction list; reduction over groups by xor shuffles.
+// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8) and every XCD has its
+// own L2.  Neighbouring work items read much of the same data (the same source boxes, the same
+// children, the same cells), so each XCD takes a contiguous range of the grid's work: workgroup
+// b handles item xcd_block().  A bijection of [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const int64_t n = gridDim.x, b = blockIdx.x;
+    const int64_t x = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + i;
+}
+
 __device__ __forceinline__ int pow2_at_least(int n)
 {
     int p = 1;
@@ -690,7 +700,7 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
                                              const double* __restrict__ tpts,
                                              Cx<R>* __restrict__ result, KParams kp)
 {
-    const int64_t it = blockIdx.x;
+    const int64_t it = xcd_block();
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -740,7 +750,7 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
     __shared__ R sx[CH], sy[CH], sz[CH];
     __shared__ R nx[CH], ny[CH], nz[CH];
     __shared__ Cx<R> sw[CH];
-    const int64_t wi = blockIdx.x;
+    const int64_t wi = xcd_block();
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -801,11 +811,11 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
                                             const ConeDev* __restrict__ cones,
                                             const BoxGeo* __restrict__ geo,
                                             const int32_t* __restrict__ child,
-                                            const int32_t* __restrict__ table, NodeTabs tabs,
+                                            const CellRef* __restrict__ ctab, NodeTabs tabs,
                                             const Cx<R>* __restrict__ F, Cx<R>* __restrict__ V,
                                             KParams kp)
 {
-    const int64_t wi = blockIdx.x;
+    const int64_t wi = xcd_block();
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -823,8 +833,8 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
                 continue;
             const BoxGeo& C = geo[ch];
             Loc L;
-            if (!locate(C, table, cones, y[0] - C.ctr[0], y[1] - C.ctr[1], y[2] - C.ctr[2],
-                        kp.f0, kp.f1, kp.f2, L))
+            if (!locate(C, ctab, y[0] - C.ctr[0], y[1] - C.ctr[1], y[2] - C.ctr[2], kp.f0,
+                        kp.f1, kp.f2, L))
                 continue;
             const Cx<R> val =
                 fine_eval<O0, O1, O2, R>(F, L.fine, kp, R(L.xi[0]), R(L.xi[1]), R(L.xi[2]));
@@ -861,13 +871,13 @@ __global__ __launch_bounds__(128) void k_up_tab(const int32_t* __restrict__ work
                                                 const ConeDev* __restrict__ cones,
                                                 const BoxGeo* __restrict__ geo,
                                                 const int32_t* __restrict__ child,
-                                                const int32_t* __restrict__ table,
+                                                const CellRef* __restrict__ ctab,
                                                 const int32_t* __restrict__ cellmap,
                                                 const UpEnt<R>* __restrict__ ents,
                                                 const Cx<R>* __restrict__ F, Cx<R>* __restrict__ V,
                                                 KParams kp)
 {
-    const int64_t wi = blockIdx.x;
+    const int64_t wi = xcd_block();
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -893,17 +903,90 @@ __global__ __launch_bounds__(128) void k_up_tab(const int32_t* __restrict__ work
             const UpEnt<R> E = e[c];
             if (E.cf < 0)
                 continue;
-            const int32_t gg = table[toff[c] + E.cf];
-            if (gg < 0)
+            const CellRef cr = ctab[toff[c] + E.cf];
+            if (!((cr.mask >> E.sub) & 1u))
                 continue;
-            const ConeDev cc = cones[gg];
-            if (!((cc.mask >> E.sub) & 1u))
-                continue;
-            const int slot = cc.fine_base + __popc(cc.mask & ((1u << E.sub) - 1u));
+            const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
             const Cx<R> val = fine_eval<O0, O1, O2, R>(F, slot, kp, E.xi0, E.xi1, E.xi2);
             acc = cadd(acc, cmul(val, Cx<R>{E.tr, E.ti}));
         }
         V[int64_t(g) * kp.n_hi + q] = acc;
+    }
+}
+
+// The same with the parent cones grouped by cell: a workgroup takes up to K parent cones of
+// one cell (item.box = the cell's index in the level's entries, glist[item.first ..] the
+// cones), so each table entry is read once for K boxes.  Children are summed in order for
+// every cone, as in k_up_tab.
+template <typename R, int O0, int O1, int O2, int K>
+__global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ items,
+                                                 int64_t nitems,
+                                                 const int32_t* __restrict__ glist,
+                                                 const ConeDev* __restrict__ cones,
+                                                 const BoxGeo* __restrict__ geo,
+                                                 const int32_t* __restrict__ child,
+                                                 const CellRef* __restrict__ ctab,
+                                                 const UpEnt<R>* __restrict__ ents,
+                                                 const Cx<R>* __restrict__ F,
+                                                 Cx<R>* __restrict__ V, KParams kp)
+{
+    __shared__ int64_t toff[K][8]; // table offset of child c of cone k; -1: no child
+    __shared__ int32_t gs[K];
+    const int64_t it = xcd_block();
+    if (it >= nitems)
+        return;
+    const WorkItem wi = items[it];
+    const int tid = threadIdx.x;
+    if (tid < K * 8)
+    {
+        const int k = tid >> 3, c = tid & 7;
+        int64_t off = -1;
+        if (k < wi.count)
+        {
+            const int32_t g = glist[wi.first + k];
+            if (c == 0)
+                gs[k] = g;
+            const int32_t ch = child[8 * cones[g].box + c];
+            if (ch >= 0)
+                off = geo[ch].table_off;
+        }
+        toff[k][c] = off;
+    }
+    __syncthreads();
+    const int n = wi.count;
+    for (int q = tid; q < kp.n_hi; q += blockDim.x)
+    {
+        const UpEnt<R>* e = ents + (int64_t(wi.box) * kp.n_hi + q) * 8;
+        Cx<R> acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            acc[k] = {0, 0};
+#pragma unroll 1
+        for (int c = 0; c < 8; c++)
+        {
+            const UpEnt<R> E = e[c];
+            if (E.cf < 0)
+                continue;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                if (k >= n)
+                    continue;
+                const int64_t to = toff[k][c];
+                if (to < 0)
+                    continue;
+                const CellRef cr = ctab[to + E.cf];
+                if (!((cr.mask >> E.sub) & 1u))
+                    continue;
+                const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
+                const Cx<R> val = fine_eval<O0, O1, O2, R>(F, slot, kp, E.xi0, E.xi1, E.xi2);
+                acc[k] = cadd(acc[k], cmul(val, Cx<R>{E.tr, E.ti}));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (k < n)
+                V[int64_t(gs[k]) * kp.n_hi + q] = acc[k];
     }
 }
 
@@ -932,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     const int tid = threadIdx.x;
     for (int i = tid; i < NW0 + NW1 + NW2; i += 256)
         Ws[i] = W[i];
-    const int64_t first = c0 + int64_t(blockIdx.x) * CPB;
+    const int64_t first = c0 + xcd_block() * CPB;
     const int ncone = int(min<int64_t>(CPB, c1 - first));
     for (int e = tid; e < ncone * NH; e += 256)
     {
@@ -1054,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
     Cx<R>* Y = X + int64_t(cpb) * xsz;
     for (int i = threadIdx.x; i < nW; i += blockDim.x)
         Ws[i] = W[i];
-    const int64_t first = c0 + int64_t(blockIdx.x) * cpb;
+    const int64_t first = c0 + xcd_block() * cpb;
     const int ncone = int(min<int64_t>(cpb, c1 - first));
     const int nh = h0 * h1 * h2;
     for (int e = threadIdx.x; e < ncone * nh; e += blockDim.x)
@@ -1133,24 +1216,32 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
 }
 
 // far field: for each target of the item, the interpolants of its far source boxes times the
-// centre factor (ifgfoperator.hpp:962-1012, 1708-1891)
-template <bool PH, typename R, int O0, int O1, int O2>
+// centre factor (ifgfoperator.hpp:962-1012, 1708-1891).  FAST (mixed form): cones located by
+// locate_fast with the level's fine-grid angle tables (doubles [ang_base, ang_base + ang_n) of
+// ang, shared by every source box of the level) staged in LDS; otherwise the exact locate.
+template <bool PH, typename R, int O0, int O1, int O2, bool FAST>
 __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, int64_t nitems,
                                             const int32_t* __restrict__ far_off,
                                             const int32_t* __restrict__ far_src,
                                             const BoxGeo* __restrict__ geo,
-                                            const int32_t* __restrict__ table,
-                                            const ConeDev* __restrict__ cones,
+                                            const CellRef* __restrict__ ctab,
                                             const Cx<R>* __restrict__ F,
                                             const double* __restrict__ tpts,
-                                            const double* __restrict__ ang,
-                                            Cx<R>* __restrict__ result, KParams kp)
+                                            const double* __restrict__ ang, int ang_base,
+                                            int ang_n, Cx<R>* __restrict__ result, KParams kp)
 {
-    const int64_t it = blockIdx.x;
+    extern __shared__ double sang[];
+    const int64_t it = xcd_block();
     if (it >= nitems)
         return;
-    const WorkItem wi = items[it];
     const int lane = threadIdx.x;
+    if constexpr (FAST)
+    {
+        for (int i = lane; i < ang_n; i += 64)
+            sang[i] = ang[ang_base + i];
+        __syncthreads();
+    }
+    const WorkItem wi = items[it];
     const int Tp = pow2_at_least(wi.count);
     const int ti = lane & (Tp - 1);
     const int grp = lane / Tp, ngrp = 64 / Tp;
@@ -1164,11 +1255,13 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
         int fine;
         double r;
         R x0, x1, x2;
-        if constexpr (sizeof(R) == 4)
+        if constexpr (FAST)
         {
             LocF L;
-            if (!locate_fast(B, ang, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2],
-                             kp.f0, kp.f1, kp.f2, L))
+            const double* eth = sang + (2 * B.fth_off - ang_base);
+            const double* eph = sang + (2 * B.fph_off - ang_base);
+            if (!locate_fast(B, eth, eph, ctab, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0,
+                             kp.f1, kp.f2, L))
                 return;
             fine = L.fine;
             r = L.r;
@@ -1179,22 +1272,21 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
         else
         {
             Loc L;
-            if (!locate(B, table, cones, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0,
-                        kp.f1, kp.f2, L))
+            if (!locate(B, ctab, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0, kp.f1, kp.f2,
+                        L))
                 return;
             fine = L.fine;
             r = L.r;
-            x0 = L.xi[0];
-            x1 = L.xi[1];
-            x2 = L.xi[2];
+            x0 = R(L.xi[0]);
+            x1 = R(L.xi[1]);
+            x2 = R(L.xi[2]);
         }
-        const Loc L{fine, r, {0, 0, 0}};
-        const Cx<R> val = fine_eval<O0, O1, O2, R>(F, L.fine, kp, x0, x1, x2);
+        const Cx<R> val = fine_eval<O0, O1, O2, R>(F, fine, kp, x0, x1, x2);
         Cx<R> cf;
         if constexpr (!PH)
-            cf = {R(INV_4PI_D / L.r), R(0)};
+            cf = {R(INV_4PI_D / r), R(0)};
         else
-            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, L.r), R(INV_4PI_D / L.r));
+            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, r), R(INV_4PI_D / r));
         acc = cadd(acc, cmul(val, cf));
     };
     if (ngrp == 1)
@@ -1289,37 +1381,54 @@ int order_class(const DevicePlan& D)
     return 0;
 }
 
+// up_level >= 0: the upward pass into parent level up_level; otherwise the far field of
+// target level far_level
 template <bool PH, typename R, int O0, int O1, int O2>
 void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs,
-                     int up_level, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
+                     int up_level, int far_level, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V,
+                     Cx<R>* res)
 {
     const bool up = up_level >= 0;
     if (up && D.up_tab[up_level] >= 0)
-        k_up_tab<R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
-            D.up_cones + a, b - a, D.cones, D.geo, D.child, D.table,
-            D.up_cellmap + D.up_map_off[up_level],
+    {
+        const int64_t i0 = D.up_item_off[up_level], i1 = D.up_item_off[up_level + 1];
+        k_up_cell<R, O0, O1, O2, UP_K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
+            D.up_items + i0, i1 - i0, D.up_cones + a, D.cones, D.geo, D.child, D.ctab,
             static_cast<const UpEnt<R>*>(D.up_entries) + D.up_ent_off[up_level], F, V, kp);
+    }
     else if (up)
         k_up<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
-            D.up_cones + a, b - a, D.cones, D.geo, D.child, D.table, tabs, F, V, kp);
+            D.up_cones + a, b - a, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
     else
-        k_far<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
-            D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.table, D.cones, F, D.tgt_pts,
-            D.fine_ang, res, kp);
+    {
+        const int abase = D.far_ang_base[far_level], an = D.far_ang_n[far_level];
+        if constexpr (sizeof(R) == 4)
+            if (an > 0)
+            {
+                k_far<PH, R, O0, O1, O2, true>
+                    <<<dim3(unsigned(b - a)), dim3(64), size_t(an) * sizeof(double), st>>>(
+                        D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F,
+                        D.tgt_pts, D.fine_ang, abase, an, res, kp);
+                return;
+            }
+        k_far<PH, R, O0, O1, O2, false><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
+            D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts,
+            D.fine_ang, 0, 0, res, kp);
+    }
 }
 
 template <bool PH, typename R>
 void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs, int up,
-                   int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
+                   int far, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
 {
     switch (order_class(D))
     {
     case 1:
-        return launch_up_far_t<PH, R, 2, 3, 3>(D, st, kp, tabs, up, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 2, 3, 3>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 2:
-        return launch_up_far_t<PH, R, 2, 5, 5>(D, st, kp, tabs, up, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 2, 5, 5>(D, st, kp, tabs, up, far, a, b, F, V, res);
     default:
-        return launch_up_far_t<PH, R, 0, 0, 0>(D, st, kp, tabs, up, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 0, 0, 0>(D, st, kp, tabs, up, far, a, b, F, V, res);
     }
 }
 
@@ -1365,7 +1474,8 @@ void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t 
 }
 
 template <int KIND, typename R>
-void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed)
+void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed,
+              int io)
 {
     constexpr bool PH = KIND != KL;
     const KParams kp = kparams(D);
@@ -1384,8 +1494,11 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
             HIP_OK(hipEventRecord(ev[i], st));
     };
     mark(0);
+    const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
+    const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
+    const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
     k_gather<R><<<dim3(unsigned((D.nsrc + 255) / 256)), dim3(256), 0, st>>>(w_dev, D.src_perm, ws,
-                                                                           D.nsrc);
+                                                                           D.nsrc, in_im);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
     if (D.n_near_items)
         k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
@@ -1413,7 +1526,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
             HIP_OK(hipEventRecord(a, st));
         }
         if (u1 > u0)
-            launch_up_far<PH, R>(D, st, kp, tabs, l, u0, u1, F, V, res);
+            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, F, V, res);
         if (timed)
             HIP_OK(hipEventRecord(b, st));
         if (c1 > c0)
@@ -1437,11 +1550,11 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     {
         const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
         if (b > a)
-            launch_up_far<PH, R>(D, st, kp, tabs, -1, a, b, F, V, res);
+            launch_up_far<PH, R>(D, st, kp, tabs, -1, int(l), a, b, F, V, res);
     }
     mark(4);
-    k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.tgt_perm,
-                                                                            out_dev, D.ntgt);
+    k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(
+        res, D.tgt_perm, out_dev, D.ntgt, out_re, out_im);
     mark(5);
     HIP_OK(hipGetLastError());
     if (timed)
@@ -1463,22 +1576,23 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     }
 }
 
-template <typename R> void dispatch(DevicePlan& D, const double* w, double* o, hipStream_t st, bool timed)
+template <typename R>
+void dispatch(DevicePlan& D, const double* w, double* o, hipStream_t st, bool timed, int io)
 {
     switch (D.kind)
     {
     case KIND_HELMHOLTZ:
-        return run_mult<KH, R>(D, w, o, st, timed);
+        return run_mult<KH, R>(D, w, o, st, timed, io);
     case KIND_MODHELMHOLTZ:
-        return run_mult<KM, R>(D, w, o, st, timed);
+        return run_mult<KM, R>(D, w, o, st, timed, io);
     case KIND_LAPLACE:
-        return run_mult<KL, R>(D, w, o, st, timed);
+        return run_mult<KL, R>(D, w, o, st, timed, io);
     case KIND_DOUBLELAYER:
-        return run_mult<KD, R>(D, w, o, st, timed);
+        return run_mult<KD, R>(D, w, o, st, timed, io);
     case KIND_COMBINED:
-        return run_mult<KC, R>(D, w, o, st, timed);
+        return run_mult<KC, R>(D, w, o, st, timed, io);
     case KIND_GRAD:
-        return run_mult<KG, R>(D, w, o, st, timed);
+        return run_mult<KG, R>(D, w, o, st, timed, io);
     }
     throw std::runtime_error("unknown kernel kind");
 }
@@ -1496,8 +1610,9 @@ bool device_available()
 void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
-                    D.src_end, D.child, D.geo, D.cones, D.table, D.ho_nodes, D.trig_th,
-                    D.trig_ph, D.fine_ang, D.wmat, D.up_cellmap, D.up_entries, D.leaf_cones, D.up_cones, D.far_off, D.far_src,
+                    D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
+                    D.trig_ph, D.fine_ang, D.wmat, D.up_cellmap, D.up_entries, D.up_items,
+                    D.leaf_cones, D.up_cones, D.far_off, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F};
     for (void* p : ptrs)
@@ -1645,7 +1760,13 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             cones[c] = {P.cone_box[c], int32_t(P.cone_cell[c]), P.cone_mask[c],
                         int32_t(P.cone_fine[c])};
         D.cones = dupload(cones, tot);
-        D.table = dupload(P.table, tot);
+        {
+            std::vector<CellRef> ct(P.table.size(), CellRef{0, 0u});
+            for (size_t i = 0; i < P.table.size(); i++)
+                if (P.table[i] >= 0)
+                    ct[i] = {cones[P.table[i]].fine_base, cones[P.table[i]].mask};
+            D.ctab = dupload(ct, tot);
+        }
         {
             auto r = chebnodes1d(D.ho[0]), t = chebnodes1d(D.ho[1]), p = chebnodes1d(D.ho[2]);
             std::vector<double> nodes;
@@ -1723,6 +1844,34 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                     (P.S.leaf[g] ? leaf : up).push_back(int32_t(c));
             D.up_off[l + 1] = int64_t(up.size());
         }
+        // levels with upward tables: parent cones grouped by cell (boxes in order within a
+        // cell), in items of at most UP_K cones of one cell
+        std::vector<WorkItem> uitems;
+        D.up_item_off.assign(D.levels + 1, 0);
+        for (int l = 0; l < D.levels; l++)
+        {
+            D.up_item_off[l] = int64_t(uitems.size());
+            const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
+            if (D.up_tab[l] < 0 || u1 == u0)
+                continue;
+            const std::vector<int32_t>& cm = P.up[l].cellmap;
+            auto cu = [&](int32_t c) { return cm[P.cone_cell[c]]; };
+            std::stable_sort(up.begin() + u0, up.begin() + u1,
+                             [&](int32_t a, int32_t b) { return cu(a) < cu(b); });
+            for (int64_t i = u0; i < u1;)
+            {
+                const int32_t u = cu(up[i]);
+                if (u < 0)
+                    throw std::runtime_error("upward table has no entry for an active cell");
+                int64_t j = i;
+                while (j < u1 && j - i < UP_K && cu(up[j]) == u)
+                    j++;
+                uitems.push_back({u, int32_t(i - u0), int32_t(j - i), 0});
+                i = j;
+            }
+        }
+        D.up_item_off[D.levels] = int64_t(uitems.size());
+        D.up_items = dupload(uitems, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
         D.up_cones = dupload(up, tot);
@@ -1755,6 +1904,8 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         // far work items by target level (one launch per level: no target is written twice)
         std::vector<WorkItem> fitems;
         D.far_item_off.assign(1, 0);
+        D.far_ang_base.clear();
+        D.far_ang_n.clear();
         for (int l = 0; l < P.T.levels; l++)
         {
             std::vector<int32_t> bl;
@@ -1764,6 +1915,26 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             auto it = make_items(bl, P.T);
             fitems.insert(fitems.end(), it.begin(), it.end());
             D.far_item_off.push_back(int64_t(fitems.size()));
+            // fine-grid angle tables of the level's far sources: staged in LDS by the mixed
+            // far kernel when every source box shares one (LDS budget 32 KB)
+            int32_t th = -1, ph = -1, n2 = -1;
+            bool uniform = true;
+            for (int32_t t : bl)
+                for (int32_t j = foff[t]; j < foff[t + 1] && uniform; j++)
+                {
+                    const BoxGeo& B = geo[fsrc[j]];
+                    if (th < 0)
+                    {
+                        th = B.fth_off;
+                        ph = B.fph_off;
+                        n2 = B.nf[2];
+                    }
+                    uniform = B.fth_off == th && B.fph_off == ph && B.nf[2] == n2;
+                }
+            const int n = th < 0 ? 0 : 2 * (ph - th) + 2 * (2 * n2 + 1);
+            const bool fits = uniform && th >= 0 && n * sizeof(double) <= 32 * 1024;
+            D.far_ang_base.push_back(fits ? 2 * th : 0);
+            D.far_ang_n.push_back(fits ? n : 0);
         }
         D.far_items = dupload(fitems, tot);
         auto nitems = make_items(ntb, P.T);
@@ -1786,7 +1957,7 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
 }
 
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
-                        void* stream)
+                        void* stream, int io)
 {
     try
     {
@@ -1812,9 +1983,9 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 odev = obuf;
             }
             if (D.prec == PREC_FP64)
-                dispatch<double>(D, wdev, odev, st, timed);
+                dispatch<double>(D, wdev, odev, st, timed, io);
             else
-                dispatch<float>(D, wdev, odev, st, timed);
+                dispatch<float>(D, wdev, odev, st, timed, io);
             if (!device_ptrs)
                 HIP_OK(hipMemcpyAsync(orr, obuf, 16 * size_t(D.ntgt), hipMemcpyDeviceToHost, st));
         }

@@ -42,6 +42,14 @@ struct ConeDev
     int32_t fine_base; // first fine slot
 };
 
+// coarse cell of a box grid, as the cone lookups read it: the fine slots of its cone in one
+// load (mask 0: no cone in the cell)
+struct CellRef
+{
+    int32_t fine_base;
+    uint32_t mask;
+};
+
 struct WorkItem // a chunk of at most 64 consecutive target points of one target box
 {
     int32_t box;   // target box (global id in the target tree)
@@ -74,7 +82,7 @@ struct DevicePlan
     int32_t* child = nullptr;     // 8 per source box
     BoxGeo* geo = nullptr;
     ConeDev* cones = nullptr;
-    int32_t* table = nullptr;
+    CellRef* ctab = nullptr;      // per box: dense coarse-cell table (BoxGeo::table_off)
     double* ho_nodes = nullptr;   // n_hi x 3 reference nodes in [-1,1]^3
     double* trig_th = nullptr;    // (cos, sin) of theta at coarse cell / node, per grid class
     double* trig_ph = nullptr;    // (cos, sin) of phi
@@ -89,6 +97,9 @@ struct DevicePlan
     std::vector<int64_t> up_map_off, up_ent_off;
     std::vector<int> up_tab;
 
+    WorkItem* up_items = nullptr;            // table levels: (cell, cones) items
+    std::vector<int64_t> up_item_off;        // per level, into up_items
+
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
     int32_t* up_cones = nullptr;             // non-leaf cones, level-major
@@ -101,6 +112,7 @@ struct DevicePlan
     int32_t* near_src = nullptr;
     WorkItem* far_items = nullptr;
     std::vector<int64_t> far_item_off; // per target level
+    std::vector<int> far_ang_base, far_ang_n; // per target level: angle tables staged in LDS
     WorkItem* near_items = nullptr;
     int64_t n_near_items = 0;
 
@@ -115,9 +127,16 @@ struct DevicePlan
 std::string device_upload(const Plan& P, int precision, DevicePlan& D);
 void device_free(DevicePlan& D);
 // w: complex128 (host or device), out: complex128 (host or device); nrhs columns of length
-// N / M stored one after another.
+// N / M stored one after another.  io: IO_* bits applied while the densities are gathered and
+// the results scattered (conjugations and a sign, for adjoint products).
+enum
+{
+    IO_CONJ_IN = 1,
+    IO_CONJ_OUT = 2,
+    IO_NEGATE_OUT = 4
+};
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
-                        void* stream);
+                        void* stream, int io = 0);
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
 void device_last_timings(const DevicePlan& D, double* out6);
 bool device_available();

@@ -81,37 +81,46 @@ class _IfgfOperator:
 
     # IfgfOperator::mult (ifgfoperator.hpp:246-867)
     def mult(self, weights):
+        return self._apply("ifgf_mult", weights, self._n, self._m)
+
+    def mult_adjoint(self, values):
+        """A^H g: the adjoint product, M values -> N (not in the reference API).  For the
+        potentials and the gradient kernel; targets other than the sources cost a transposed
+        plan on the first call."""
+        return self._apply("ifgf_mult_adjoint", values, self._m, self._n)
+
+    def _apply(self, fname, weights, n_in, n_out):
         if self._n is None:
             raise _lib.IfgfError("mult before init")
         try:
             import torch
             if isinstance(weights, torch.Tensor) and weights.is_cuda:
-                return self._mult_torch(weights)
+                return self._apply_torch(fname, weights, n_in, n_out)
         except ImportError:
             pass
         w = np.asarray(weights)
         squeeze = w.ndim == 1
-        w2 = w.reshape(self._n, -1) if not squeeze else w.reshape(self._n, 1)
+        w2 = w.reshape(n_in, -1) if not squeeze else w.reshape(n_in, 1)
         nrhs = w2.shape[1]
         wc = np.ascontiguousarray(w2.T.astype(np.complex128))  # nrhs vectors back to back
-        out = np.empty((nrhs, self._m), dtype=np.complex128)
-        check(lib().ifgf_mult(self._h, wc.ctypes.data, nrhs, out.ctypes.data, 0, None),
-              self._h, "mult")
+        out = np.empty((nrhs, n_out), dtype=np.complex128)
+        check(getattr(lib(), fname)(self._h, wc.ctypes.data, nrhs, out.ctypes.data, 0, None),
+              self._h, fname[5:])
         res = out.T
         if self._real:
             res = res.real.copy()
         return res[:, 0].copy() if squeeze else np.ascontiguousarray(res)
 
-    def _mult_torch(self, w):
+    def _apply_torch(self, fname, w, n_in, n_out):
         import torch
         squeeze = w.dim() == 1
-        w2 = w.reshape(self._n, 1) if squeeze else w.reshape(self._n, -1)
+        w2 = w.reshape(n_in, 1) if squeeze else w.reshape(n_in, -1)
         nrhs = w2.shape[1]
         wc = w2.t().to(torch.complex128).contiguous()
-        out = torch.empty((nrhs, self._m), dtype=torch.complex128, device=w.device)
+        out = torch.empty((nrhs, n_out), dtype=torch.complex128, device=w.device)
         stream = torch.cuda.current_stream(w.device).cuda_stream
-        check(lib().ifgf_mult(self._h, wc.data_ptr(), nrhs, out.data_ptr(),
-                              _lib.DEVICE_POINTERS, stream), self._h, "mult")
+        check(getattr(lib(), fname)(self._h, wc.data_ptr(), nrhs, out.data_ptr(),
+                                    _lib.DEVICE_POINTERS, stream), self._h, fname[5:])
         res = out.t()
         if self._real:
             res = res.real

@@ -69,6 +69,17 @@ int ifgf_init(ifgf_operator* op, const double* srcs, int64_t n_src, const double
 int ifgf_mult(ifgf_operator* op, const double* w, int64_t nrhs, double* out, int flags,
               void* stream);
 
+/* The adjoint product A^H g of mult (out: N values per column, g: M).  No counterpart in the
+ * reference, which has no adjoint; it serves the autograd wrapper's backward (the gradient
+ * with respect to the densities).  For the potentials G(x - y) = G(y - x) and for the
+ * gradient kernel, odd in x - y, A^H g = (+-) conj(B conj(g)) with B the IFGF operator from the
+ * targets to the sources: this plan when the targets are the sources, else a transposed plan
+ * built and uploaded on the first call (the cutoff callback of init must still be valid).
+ * The double-layer and combined-field kernels return IFGF_EINVAL (their adjoints need
+ * target normals).  Flags and stream as for ifgf_mult. */
+int ifgf_mult_adjoint(ifgf_operator* op, const double* g, int64_t nrhs, double* out, int flags,
+                      void* stream);
+
 void ifgf_destroy(ifgf_operator* op);
 const char* ifgf_last_error(const ifgf_operator* op);
 const char* ifgf_version(void);

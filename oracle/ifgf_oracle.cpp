@@ -1143,6 +1143,18 @@ struct Operator
             estimate_refinement();
         else
             base_order[0] = std::max(base_order[0] - 3, 2);
+        // the unclamped fine order (base - 3) of Laplace, modified Helmholtz, double layer and
+        // combined field leaves interpolants without coefficients for small orders
+        // (laplace_ifgf.hpp:133-145): the reference's behaviour is undefined there; refuse it
+        for (int step = 0; step < 2; step++)
+        {
+            int o[3];
+            K.order_for_box(1.0, base_order, step, o);
+            for (int d = 0; d < 3; d++)
+                if (o[d] < 1)
+                    throw std::runtime_error("interpolation order below 1 (the fine order of "
+                                             "this kernel is order-3)");
+        }
         // RECURSIVE_MULT level choice (octree.hpp:372-380; ifgfoperator.hpp:292-300)
         min_rec = 0;
         for (; min_rec < S.levels(); min_rec++)

@@ -33,6 +33,30 @@ def spheroid(n, seed, c=0.5):
     return np.concatenate(out)[:n]
 
 
+def perturbed_sphere(n, seed, lmax=8, amp=0.1):
+    """r = 1 + amp sum_{l<=lmax, |m|<=l} a_lm Y_lm(theta, phi) with real spherical harmonics
+    and a_lm ~ N(0,1)/(l+1), at n random directions (the perturbed-sphere geometry)."""
+    from scipy.special import sph_harm_y
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((n, 3))
+    p = g / np.linalg.norm(g, axis=1)[:, None]
+    theta = np.arccos(np.clip(p[:, 2], -1.0, 1.0))
+    phi = np.arctan2(p[:, 1], p[:, 0])
+    r = np.ones(n)
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            a = rng.standard_normal() / (l + 1)
+            y = sph_harm_y(l, abs(m), theta, phi)
+            if m > 0:
+                yr = np.sqrt(2.0) * (-1) ** m * y.real
+            elif m < 0:
+                yr = np.sqrt(2.0) * (-1) ** m * y.imag
+            else:
+                yr = y.real
+            r += amp * a * yr
+    return p * r[:, None]
+
+
 def cdens(n, seed):
     r = np.random.default_rng(seed)
     return r.standard_normal(n) + 1j * r.standard_normal(n)

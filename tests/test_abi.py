@@ -37,6 +37,7 @@ def test_version_and_errors_without_device():
     assert L.ifgf_create(0, 1.0, 0.0, 64, 6, 1, -1.0, 1, ctypes.byref(h)) == _lib.IFGF_OK
     w = np.zeros(8, complex)
     assert L.ifgf_mult(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
+    assert L.ifgf_mult_adjoint(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
     assert L.ifgf_set_dx(h, 1) == _lib.IFGF_EINVAL  # not a gradient operator
     if L.ifgf_device_count() == 0:
         x = np.random.default_rng(0).standard_normal((50, 3))

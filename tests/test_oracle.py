@@ -246,16 +246,16 @@ def test_kernel_formulas_and_derivatives():
     o = po.Oracle(po.HELMHOLTZ, kap, 64, 6).init(pts, pts)
     assert np.isclose(o.kernel(xs, y), np.exp(1j * kap * d) / (4 * np.pi * d), rtol=1e-14)
     assert o.kernel(xs, xs) == 0  # G(0) = 0 (helmholtz_ifgf.hpp:39-40)
-    o = po.Oracle(po.MODHELMHOLTZ, k, 64, 6).init(pts, pts)
+    o = po.Oracle(po.MODHELMHOLTZ, k, 64, 8).init(pts, pts)
     assert np.isclose(o.kernel(xs, y), _G(k, d), rtol=1e-14)
-    o = po.Oracle(po.LAPLACE, 0, 64, 6).init(pts, pts)
+    o = po.Oracle(po.LAPLACE, 0, 64, 8).init(pts, pts)
     assert np.isclose(o.kernel(xs, y).real, 1 / (4 * np.pi * d), rtol=1e-14)
     # double layer = -d/dt G(x_src + t n - y)   (double_layer_helmholtz_ifgf.hpp:58-66)
-    o = po.Oracle(po.DOUBLELAYER, k, 64, 6).init(pts, pts, normals=np.stack([n, n]))
+    o = po.Oracle(po.DOUBLELAYER, k, 64, 8).init(pts, pts, normals=np.stack([n, n]))
     fd = (_G(k, np.linalg.norm(xs + h * n - y)) - _G(k, np.linalg.norm(xs - h * n - y))) / (2 * h)
     assert np.isclose(o.kernel(xs, y, 0), -fd, rtol=1e-6)
     # combined field = d/dn_src G - i kappa G with G = e^{i kappa d}/(4 pi d)
-    o = po.Oracle(po.COMBINED, kap, 64, 6).init(pts, pts, normals=np.stack([n, n]))
+    o = po.Oracle(po.COMBINED, kap, 64, 8).init(pts, pts, normals=np.stack([n, n]))
     kk = -1j * kap
     fd = (_G(kk, np.linalg.norm(xs + h * n - y)) - _G(kk, np.linalg.norm(xs - h * n - y))) / (2 * h)
     assert np.isclose(o.kernel(xs, y, 0), fd - 1j * kap * _G(kk, d), rtol=1e-6)

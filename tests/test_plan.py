@@ -144,6 +144,14 @@ def test_plan_errors():
     op = ifgf_amd.MofifiedHelmholtzIfgfOperator(1 + 10j, 64, 3, 1, -1)
     with pytest.raises(ifgf_amd._lib.IfgfError):  # fine order = 3 - 3 = 0 (no clamp)
         op.init(sphere(500, 1).T, sphere(500, 1).T, plan_only=True)
+    # Laplace at order 6: radial base order 3, fine order 0 (laplace_ifgf.hpp:133-145); the
+    # library and the oracle both refuse it
+    x = sphere(500, 1)
+    with pytest.raises(ifgf_amd._lib.IfgfError):
+        ifgf_amd.LaplaceIfgfOperator(64, 6, 1, -1).init(x.T, x.T, plan_only=True)
+    with pytest.raises(RuntimeError, match="order below 1"):
+        po.Oracle(po.LAPLACE, 0.0, 64, 6, 1, -1).init(x, x)
+    ifgf_amd.LaplaceIfgfOperator(64, 7, 1, -1).init(x.T, x.T, plan_only=True)
     op = ifgf_amd.HelmholtzIfgfOperator(1.0, 64, 6, 1, -1)
     with pytest.raises(ValueError):
         op.init(np.zeros((2, 5)), np.zeros((2, 5)), plan_only=True)
