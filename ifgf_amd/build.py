@@ -22,15 +22,24 @@ def _run(cmd):
 
 
 def build(verbose=True):
+    """Compile every translation unit (in parallel) and link libifgf_amd.so."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src in ("plan.cpp", "capi.cpp"):
-        obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
-    obj = os.path.join(BUILD, "device.hip.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "device.hip"), "-o", obj])
-    objs.append(obj)
+        jobs.append((["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src)],
+                     os.path.join(BUILD, src + ".o")))
+    for src in ("device.hip", "batched.hip"):
+        jobs.append(([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src)],
+                     os.path.join(BUILD, src + ".o")))
+    procs = []
+    for cmd, obj in jobs:
+        full = cmd + ["-o", obj]
+        print(" ".join(full), flush=True)
+        procs.append((full, subprocess.Popen(full)))
+    failed = [full for full, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-pthread"])
     os.replace(tmp, LIB)

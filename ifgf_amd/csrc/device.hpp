@@ -116,16 +116,28 @@ struct DevicePlan
     WorkItem* near_items = nullptr;
     int64_t n_near_items = 0;
 
-    // work buffers (resized per nrhs)
+    // work buffers of the single-column mult
     void* wsorted = nullptr;  // N complex (R)
     void* result = nullptr;   // M complex (R)
     void* V = nullptr;        // n_cones * n_hi complex (R)
     void* F = nullptr;        // n_fine * n_lo complex (R)
     int64_t bytes_total = 0;
+
+    // work buffers of the batched mult (b_rb columns, column-minor), allocated on first use;
+    // b_in / b_out stage host densities and results
+    int b_rb = 0;
+    void *b_ws = nullptr, *b_res = nullptr, *b_V = nullptr, *b_F = nullptr;
+    double *b_in = nullptr, *b_out = nullptr;
+    int64_t b_bytes = 0;
 };
 
 std::string device_upload(const Plan& P, int precision, DevicePlan& D);
 void device_free(DevicePlan& D);
+// batched mult (batched.hip): ncol <= rb columns (rb = 8 or 32) of device densities with the
+// batch buffers allocated for rb; orders 6 and 8 with tol <= 0 only (batched_supported)
+bool batched_supported(const DevicePlan& D);
+void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, int ncol,
+                  void* stream, int io);
 // w: complex128 (host or device), out: complex128 (host or device); nrhs columns of length
 // N / M stored one after another.  io: IO_* bits applied while the densities are gathered and
 // the results scattered (conjugations and a sign, for adjoint products).

@@ -116,7 +116,9 @@ class _IfgfOperator:
         squeeze = w.dim() == 1
         w2 = w.reshape(n_in, 1) if squeeze else w.reshape(n_in, -1)
         nrhs = w2.shape[1]
-        wc = w2.t().to(torch.complex128).contiguous()
+        # conj() and neg() of complex tensors are lazy views: materialise them before the
+        # raw pointer goes to the library
+        wc = w2.t().to(torch.complex128).resolve_conj().resolve_neg().contiguous()
         out = torch.empty((nrhs, n_out), dtype=torch.complex128, device=w.device)
         stream = torch.cuda.current_stream(w.device).cuda_stream
         check(getattr(lib(), fname)(self._h, wc.data_ptr(), nrhs, out.data_ptr(),
