@@ -82,8 +82,8 @@ __device__ __forceinline__ Cx<R> fine_eval_s(const Cx<R>* __restrict__ F, int sl
     return clen3c<O0, O1, O2, R>(c, x0, x1, x2);
 }
 
-// near field (ifgfoperator.hpp:900-960): lanes = targets as in the single-column kernel, the
-// kernel value of each (source, target) pair applied to the RB columns
+// near field (ifgfoperator.hpp:900-960): as k_near (source leaves staged in LDS, lanes =
+// (target, group)), the kernel value of each (source, target) pair applied to the RB columns
 template <int KIND, typename R, int RB>
 __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ items,
                                                int64_t nitems,
@@ -97,6 +97,10 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
                                                const double* __restrict__ tpts,
                                                Cx<R>* __restrict__ result, KParams kp)
 {
+    constexpr bool NRM = KIND == KD || KIND == KC;
+    __shared__ double sp[3 * 64];
+    __shared__ double sn[NRM ? 3 * 64 : 1];
+    __shared__ Cx<R> sw[64 * RB];
     const int64_t it = xcd_block();
     if (it >= nitems)
         return;
@@ -113,20 +117,40 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
     for (int r = 0; r < RB; r++)
         acc[r] = {0, 0};
     const int32_t j0 = near_off[wi.box], j1 = near_off[wi.box + 1];
-    for (int32_t j = j0 + grp; j < j1; j += ngrp)
+    for (int32_t j = j0; j < j1; j++)
     {
         const int32_t s = near_src[j];
         const int64_t b = sbeg[s], e = send[s];
-        for (int64_t i = b; i < e; i++)
+        for (int64_t c0 = b; c0 < e; c0 += 64)
         {
-            const R x0 = R(spts[3 * i] - y0), x1 = R(spts[3 * i + 1] - y1),
-                    x2 = R(spts[3 * i + 2] - y2);
-            const Cx<R> g =
-                p2p<KIND, R>(kp, x0, x1, x2, snrm ? snrm + 3 * i : nullptr, Cx<R>{1, 0});
-            const Cx<R>* wr = w + i * RB;
+            const int n = int(min<int64_t>(64, e - c0));
+            __syncthreads();
+            if (lane < n)
+            {
+                const int64_t i = c0 + lane;
+                sp[3 * lane] = spts[3 * i];
+                sp[3 * lane + 1] = spts[3 * i + 1];
+                sp[3 * lane + 2] = spts[3 * i + 2];
+                if constexpr (NRM)
+                {
+                    sn[3 * lane] = snrm[3 * i];
+                    sn[3 * lane + 1] = snrm[3 * i + 1];
+                    sn[3 * lane + 2] = snrm[3 * i + 2];
+                }
+            }
+            for (int e2 = lane; e2 < n * RB; e2 += 64)
+                sw[e2] = w[c0 * RB + e2];
+            __syncthreads();
+            for (int k = grp; k < n; k += ngrp)
+            {
+                const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
+                        x2 = R(sp[3 * k + 2] - y2);
+                const Cx<R> g =
+                    p2p<KIND, R>(kp, x0, x1, x2, NRM ? sn + 3 * k : nullptr, Cx<R>{1, 0});
 #pragma unroll
-            for (int r = 0; r < RB; r++)
-                acc[r] = cadd(acc[r], cmul(g, wr[r]));
+                for (int r = 0; r < RB; r++)
+                    acc[r] = cadd(acc[r], cmul(g, sw[k * RB + r]));
+            }
         }
     }
 #pragma unroll

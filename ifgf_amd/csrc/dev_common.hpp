@@ -500,51 +500,81 @@ __device__ __forceinline__ Cx<R> fine_eval(const Cx<R>* __restrict__ F, int slot
 // ------------------------------------------------------------------------------------
 // kernel functions.  P2P: G(x_src - y) (helmholtz_ifgf.hpp:37-41 and siblings)
 // ------------------------------------------------------------------------------------
+// distance and its inverse: in the mixed form from one v_rsq_f32 (1 ulp), in FP64 exactly
+// (sqrt, then a division).  d2 > 0.
+template <typename R> __device__ __forceinline__ void dist_inv(R d2, R& d, R& invd)
+{
+    if constexpr (sizeof(R) == 4)
+    {
+        invd = __builtin_amdgcn_rsqf(d2);
+        d = d2 * invd;
+    }
+    else
+    {
+        d = sqrt(d2);
+        invd = R(1) / d;
+    }
+}
+
+// a / b: v_rcp_f32 (1 ulp) in the mixed form, a division in FP64
+template <typename R> __device__ __forceinline__ R fdiv_r(R a, R b)
+{
+    if constexpr (sizeof(R) == 4)
+        return a * __builtin_amdgcn_rcpf(b);
+    else
+        return a / b;
+}
+
 template <int KIND, typename R>
 __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const double* nrm,
                                      Cx<R> w)
 {
     // (x0,x1,x2) = source - target
     const R d2 = x0 * x0 + x1 * x1 + x2 * x2;
-    const R d = sqrt(d2);
     const R inv4pi = R(INV_4PI_D);
     if constexpr (KIND == KH || KIND == KM || (KIND == KG))
     {
+        if (KIND == KG ? (d2 < R(1e-24)) : (d2 == R(0)))
+            return {0, 0};
+        R d, invd;
+        dist_inv(d2, d, invd);
         if (KIND == KG && kp.dx >= 0)
         {
-            if (d < R(1e-12))
-                return {0, 0};
             const R xd = kp.dx == 0 ? x0 : (kp.dx == 1 ? x1 : x2);
             Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
-            Cx<R> mk = {R(-kp.k_re) - R(1) / d, R(-kp.k_im)};
+            Cx<R> mk = {R(-kp.k_re) - invd, R(-kp.k_im)};
             Cx<R> v = cmul(e, mk);
-            R s = -inv4pi / d2 * xd;
+            R s = -inv4pi * (invd * invd) * xd;
             return cmul(w, cscale(v, s));
         }
-        if (KIND == KG ? (d < R(1e-12)) : (d == R(0)))
-            return {0, 0};
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
-        return cmul(w, cscale(e, inv4pi / d));
+        return cmul(w, cscale(e, inv4pi * invd));
     }
     else if constexpr (KIND == KL)
     {
-        if (d == R(0))
+        if (d2 == R(0))
             return {0, 0};
-        return cscale(w, inv4pi / d);
+        R d, invd;
+        dist_inv(d2, d, invd);
+        return cscale(w, inv4pi * invd);
     }
     else if constexpr (KIND == KD)
     {
-        if (d < R(1e-12))
+        if (d2 < R(1e-24))
             return {0, 0};
+        R d, invd;
+        dist_inv(d2, d, invd);
         const R xn = x0 * R(nrm[0]) + x1 * R(nrm[1]) + x2 * R(nrm[2]);
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
-        Cx<R> mk = {R(-kp.k_re) - R(1) / d, R(-kp.k_im)};
+        Cx<R> mk = {R(-kp.k_re) - invd, R(-kp.k_im)};
         Cx<R> v = cmul(e, mk);
-        return cmul(w, cscale(v, -inv4pi / d2 * xn));
+        return cmul(w, cscale(v, -inv4pi * (invd * invd) * xn));
     }
     else // KC: combined_field_helmholtz_ifgf.hpp:43-68
     {
-        const R invd = d2 < R(1e-14) ? R(0) : R(1) / d;
+        R d = 0, invd = 0;
+        if (!(d2 < R(1e-14)))
+            dist_inv(d2, d, invd);
         const R dd = d2 * invd;
         const R nxy = -(R(nrm[0]) * x0 + R(nrm[1]) * x1 + R(nrm[2]) * x2);
         const R kap = R(kp.kappa);
@@ -566,43 +596,43 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
 {
     const R z0 = x0 - y0, z1 = x1 - y1, z2 = x2 - y2; // source - node
     const R d2 = z0 * z0 + z1 * z1 + z2 * z2;
-    const R d = sqrt(d2);
+    if (d2 == R(0)) // a source on a node: the kernels vanish there (G(0) = 0)
+        return {0, 0};
+    R d, invd;
+    dist_inv(d2, d, invd);
     R delta;
     if constexpr (sizeof(R) == 4)
-        delta = (x0 * (x0 - R(2) * y0) + x1 * (x1 - R(2) * y1) + x2 * (x2 - R(2) * y2)) / (d + dc);
+        delta = fdiv_r(x0 * (x0 - R(2) * y0) + x1 * (x1 - R(2) * y1) + x2 * (x2 - R(2) * y2),
+                       d + dc);
     else
         delta = d - dc;
     if constexpr (KIND == KH || KIND == KM)
     {
-        if (d == R(0))
-            return {0, 0};
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, delta);
-        return cmul(w, cscale(e, dc / d));
+        return cmul(w, cscale(e, dc * invd));
     }
     else if constexpr (KIND == KL)
     {
-        if (d == R(0))
-            return {0, 0};
-        return cscale(w, dc / d);
+        return cscale(w, dc * invd);
     }
     else if constexpr (KIND == KG)
     {
-        const R id = d2 > R(1e-12) ? R(1) / d : R(0);
+        const R id = d2 > R(1e-12) ? invd : R(0);
         if (kp.dx < 0)
         {
             Cx<R> e = expmk<R>(kp.k_re, kp.k_im, delta);
             return cmul(w, cscale(e, dc * id));
         }
         const R zd = kp.dx == 0 ? z0 : (kp.dx == 1 ? z1 : z2);
-        const R ww = -(R(1) / d2) * dc * zd;
+        const R ww = -(invd * invd) * dc * zd;
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, delta);
         Cx<R> mk = {R(-kp.k_re) - id, R(-kp.k_im)};
         return cmul(w, cscale(cmul(e, mk), ww));
     }
     else if constexpr (KIND == KD)
     {
-        const R id = d2 > R(1e-12) ? R(1) / d : R(0);
-        const R ww = -(R(1) / d2) * dc * (z0 * nrm[0] + z1 * nrm[1] + z2 * nrm[2]);
+        const R id = d2 > R(1e-12) ? invd : R(0);
+        const R ww = -(invd * invd) * dc * (z0 * nrm[0] + z1 * nrm[1] + z2 * nrm[2]);
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, delta);
         Cx<R> mk = {R(-kp.k_re) - id, R(-kp.k_im)};
         return cmul(w, cscale(cmul(e, mk), ww));
@@ -615,7 +645,7 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
         const R kap = R(kp.kappa);
         R s, c;
         sincos_r<R>(kap * delta, &s, &c);
-        const R f = dc / (d2 * d);
+        const R f = dc * (invd * invd * invd);
         const R a = w.x * c - w.y * s, b = w.x * s + w.y * c;
         return {f * (a * nxy + b * kap * (d * nxy + d2)),
                 f * (-a * kap * (d * nxy + d2) + b * nxy)};

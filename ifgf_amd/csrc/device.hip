@@ -65,7 +65,9 @@ template <typename R> __global__ void k_zero(Cx<R>* p, int64_t n)
 
 
 // near field: for each target of the item, sum over the points of its near source leaves
-// (ifgfoperator.hpp:900-960; octree.hpp:302-307)
+// (ifgfoperator.hpp:900-960; octree.hpp:302-307).  The wave walks the source leaves in order
+// and stages each (64 points at a time) in LDS with one coalesced load; lane = (target, group),
+// the groups splitting the staged points; xor-shuffle reduction over the groups at the end.
 template <int KIND, typename R>
 __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items, int64_t nitems,
                                              const int32_t* __restrict__ near_off,
@@ -78,6 +80,10 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
                                              const double* __restrict__ tpts,
                                              Cx<R>* __restrict__ result, KParams kp)
 {
+    constexpr bool NRM = KIND == KD || KIND == KC;
+    __shared__ double sp[3 * 64];
+    __shared__ double sn[NRM ? 3 * 64 : 1];
+    __shared__ Cx<R> sw[64];
     const int64_t it = xcd_block();
     if (it >= nitems)
         return;
@@ -91,15 +97,35 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
     const double y0 = tpts[3 * t], y1 = tpts[3 * t + 1], y2 = tpts[3 * t + 2];
     Cx<R> acc = {0, 0};
     const int32_t j0 = near_off[wi.box], j1 = near_off[wi.box + 1];
-    for (int32_t j = j0 + grp; j < j1; j += ngrp)
+    for (int32_t j = j0; j < j1; j++)
     {
         const int32_t s = near_src[j];
         const int64_t b = sbeg[s], e = send[s];
-        for (int64_t i = b; i < e; i++)
+        for (int64_t c0 = b; c0 < e; c0 += 64)
         {
-            const R x0 = R(spts[3 * i] - y0), x1 = R(spts[3 * i + 1] - y1),
-                    x2 = R(spts[3 * i + 2] - y2);
-            acc = cadd(acc, p2p<KIND, R>(kp, x0, x1, x2, snrm ? snrm + 3 * i : nullptr, w[i]));
+            const int n = int(min<int64_t>(64, e - c0));
+            __syncthreads();
+            if (lane < n)
+            {
+                const int64_t i = c0 + lane;
+                sp[3 * lane] = spts[3 * i];
+                sp[3 * lane + 1] = spts[3 * i + 1];
+                sp[3 * lane + 2] = spts[3 * i + 2];
+                if constexpr (NRM)
+                {
+                    sn[3 * lane] = snrm[3 * i];
+                    sn[3 * lane + 1] = snrm[3 * i + 1];
+                    sn[3 * lane + 2] = snrm[3 * i + 2];
+                }
+                sw[lane] = w[i];
+            }
+            __syncthreads();
+            for (int k = grp; k < n; k += ngrp)
+            {
+                const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
+                        x2 = R(sp[3 * k + 2] - y2);
+                acc = cadd(acc, p2p<KIND, R>(kp, x0, x1, x2, NRM ? sn + 3 * k : nullptr, sw[k]));
+            }
         }
     }
     for (int m = Tp; m < 64; m <<= 1)
