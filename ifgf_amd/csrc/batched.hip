@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
             {
                 const BoxGeo& B = geo[__builtin_amdgcn_readfirstlane(far_src[j])];
                 bool ok = false;
-                double rr = 1;
+                double rr = 1, ri = 1;
                 if (tvalid)
                 {
                     if constexpr (FAST)
@@ -562,10 +562,11 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
                         LocF L;
                         const double* eth = sang + (2 * B.fth_off - ang_base);
                         const double* eph = sang + (2 * B.fph_off - ang_base);
-                        ok = locate_fast(B, eth, eph, ctab, y0 - B.ctr[0], y1 - B.ctr[1],
-                                         y2 - B.ctr[2], kp.f0, kp.f1, kp.f2, L);
+                        ok = locate_fast<3>(B, eth, eph, ctab, y0 - B.ctr[0], y1 - B.ctr[1],
+                                            y2 - B.ctr[2], kp.f0, kp.f1, kp.f2, L);
                         slot = L.fine;
                         rr = L.r;
+                        ri = L.rinv;
                         x0 = L.xi[0];
                         x1 = L.xi[1];
                         x2 = L.xi[2];
@@ -577,6 +578,7 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
                                     kp.f1, kp.f2, L);
                         slot = L.fine;
                         rr = L.r;
+                        ri = 1.0 / L.r;
                         x0 = R(L.xi[0]);
                         x1 = R(L.xi[1]);
                         x2 = R(L.xi[2]);
@@ -585,9 +587,9 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
                 if (ok)
                 {
                     if constexpr (!PH)
-                        cf = {R(INV_4PI_D / rr), R(0)};
+                        cf = {R(INV_4PI_D * ri), R(0)};
                     else
-                        cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, rr), R(INV_4PI_D / rr));
+                        cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, rr), R(INV_4PI_D * ri));
                 }
                 else
                     slot = -1;

@@ -115,10 +115,23 @@ template <typename R> __device__ __forceinline__ Cx<R> expmk_far(double kr, doub
         sincos(ph, &s, &c);
     if (kr != 0.0)
     {
-        R m = R(exp(-kr * r));
+        R m;
+        if constexpr (sizeof(R) == 4)
+            m = expf(float(-kr * r));
+        else
+            m = exp(-kr * r);
         return {m * c, m * s};
     }
     return {c, s};
+}
+
+// 1/sqrt(x) for x > 0 from v_rsq_f64 (about 2^-23 relative) and one Newton step (about
+// 2e-14 relative)
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-0.5 * x * y, y, 0.5);
+    return fma(y, e, y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -207,68 +220,91 @@ __device__ __forceinline__ float atan_small(float u)
 struct LocF
 {
     int fine;
-    double r;
+    double r;    // |v| (about 2e-14 relative)
+    double rinv; // 1/|v|
     float xi[3];
 };
 
+// one angle axis of locate_fast: the cell from the FP32 candidate, the local coordinate from
+// the offset rotated into the cell-centre frame (FP64 rotation, FP32 small-angle atan).  When
+// |xi| comes out within 1e-5 of 1 the point is close to an edge (the candidate's error is about
+// 1e-5 rad, far below half a cell): the cell is then decided by the FP64 cross products with the
+// edges.  E: (cos, sin) of the n + 1 edges, then of the n centres.  (p, q): the offset in the
+// axis' plane, angle = atan2(p, q).
+__device__ __forceinline__ int angle_cell(const double* E, int n, double lo_shift, double inv_h,
+                                          double p, double q, float& xi)
+{
+    const float cand = atan2_coarse(float(p), float(q)) + float(lo_shift);
+    int c = min(max(int(cand * float(inv_h)), 0), n - 1);
+    const float sc = float(2.0 * inv_h);
+    const double* Cc = E + 2 * (n + 1) + 2 * c;
+    double xr = q * Cc[0] + p * Cc[1], yr = p * Cc[0] - q * Cc[1];
+    xi = atan_small(__fdividef(float(yr), float(xr))) * sc;
+    if (fabsf(xi) >= 1.f - 1e-5f)
+    {
+        if (c > 0 && !(p * E[2 * c] - q * E[2 * c + 1] >= 0.0))
+            c--;
+        else if (c < n - 1 && p * E[2 * c + 2] - q * E[2 * c + 3] >= 0.0)
+            c++;
+        Cc = E + 2 * (n + 1) + 2 * c;
+        xr = q * Cc[0] + p * Cc[1];
+        yr = p * Cc[0] - q * Cc[1];
+        xi = atan_small(__fdividef(float(yr), float(xr))) * sc;
+    }
+    return c;
+}
+
+// FF: the fine/coarse factor of every axis when known at compile time (else f0, f1, f2)
+template <int FF = 0>
 __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* eth, const double* eph,
                                             const CellRef* __restrict__ ctab, double v0,
-                                            double v1, double v2, int f0, int f1, int f2,
+                                            double v1, double v2, int f0_, int f1_, int f2_,
                                             LocF& out)
 {
-    double a, rho, r, s;
+    const int f0 = FF ? FF : f0_, f1 = FF ? FF : f1_, f2 = FF ? FF : f2_;
+    double a, r2;
     {
 #pragma clang fp contract(off)
         a = v0 * v0 + v1 * v1;
-        rho = sqrt(a);
-        r = sqrt(a + v2 * v2);
-        s = B.H / r;
+        r2 = a + v2 * v2;
     }
-    out.r = r;
-    if (s < B.lo[0] || s > B.hi[0])
-        return false;
+    const double rinv = rsq_nr(r2);
+    double s = B.H * rinv;
+    const double rho = a > 0.0 ? a * rsq_nr(a) : 0.0;
+    out.r = r2 * rinv;
+    out.rinv = rinv;
     int q[3];
     {
-        const double d = s - B.lo[0];
-        const double t = d * B.inv_hf[0];
+        // s decides the domain and the radial cell: near a bound or a cell edge it is
+        // recomputed as the plan computes it (correctly rounded sqrt and division)
+        double d = s - B.lo[0];
+        double t = d * B.inv_hf[0];
         double fl = floor(t);
-        const double fr = t - fl;
-        if (fr < 1e-9 || fr > 1.0 - 1e-9)
-            fl = floor(d / B.hf[0]);
+        double fr = t - fl;
+        const double tol = 1e-12 * s;
+        if (fabs(s - B.lo[0]) < tol || fabs(s - B.hi[0]) < tol || fr < 1e-9 ||
+            fr > 1.0 - 1e-9)
+        {
+            s = B.H / sqrt(r2);
+            if (s < B.lo[0] || s > B.hi[0])
+                return false;
+            d = s - B.lo[0];
+            t = d * B.inv_hf[0];
+            fl = floor(t);
+            fr = t - fl;
+            if (fr < 1e-9 || fr > 1.0 - 1e-9)
+                fl = floor(d / B.hf[0]);
+        }
+        else if (s < B.lo[0] || s > B.hi[0])
+            return false;
         q[0] = min(max(int(fl), 0), B.nf[0] - 1);
         out.xi[0] = float(2.0 * (t - double(q[0])) - 1.0);
     }
     const int n1 = B.nf[1], n2 = B.nf[2];
-    {
-        // theta in [0, pi]: edges k h1 (rho = 0 included: theta = 0 or pi)
-        int c = int(atan2_coarse(float(rho), float(v2)) * float(B.inv_hf[1]));
-        c = min(max(c, 0), n1 - 1);
-        const double* E = eth;
-        if (c > 0 && !(rho * E[2 * c] - v2 * E[2 * c + 1] >= 0.0))
-            c--;
-        else if (c < n1 - 1 && rho * E[2 * c + 2] - v2 * E[2 * c + 3] >= 0.0)
-            c++;
-        q[1] = c;
-        const double* Cc = E + 2 * (n1 + 1) + 2 * c;
-        const double xr = v2 * Cc[0] + rho * Cc[1], yr = rho * Cc[0] - v2 * Cc[1];
-        out.xi[1] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[1]);
-    }
-    if (a != 0.0)
-    {
-        // phi in [-pi, pi]: edges -pi + k h2
-        const float ph = atan2_coarse(float(v1), float(v0)) + 3.141592653589793f;
-        int c = int(ph * float(B.inv_hf[2]));
-        c = min(max(c, 0), n2 - 1);
-        const double* E = eph;
-        if (c > 0 && !(v1 * E[2 * c] - v0 * E[2 * c + 1] >= 0.0))
-            c--;
-        else if (c < n2 - 1 && v1 * E[2 * c + 2] - v0 * E[2 * c + 3] >= 0.0)
-            c++;
-        q[2] = c;
-        const double* Cc = E + 2 * (n2 + 1) + 2 * c;
-        const double xr = v0 * Cc[0] + v1 * Cc[1], yr = v1 * Cc[0] - v0 * Cc[1];
-        out.xi[2] = atan_small(__fdividef(float(yr), float(xr))) * float(2.0 * B.inv_hf[2]);
-    }
+    // theta in [0, pi] (rho = 0 included: theta = 0 or pi)
+    q[1] = angle_cell(eth, n1, 0.0, B.inv_hf[1], rho, v2, out.xi[1]);
+    if (a != 0.0) // phi in [-pi, pi]
+        q[2] = angle_cell(eph, n2, PI_D, B.inv_hf[2], v1, v0, out.xi[2]);
     else
     {
         // on the polar axis phi is atan2 of signed zeros (+-0 or +-pi), placed by the same
@@ -283,13 +319,22 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* eth, 
         q[2] = min(max(int(fl), 0), n2 - 1);
         out.xi[2] = float(2.0 * (t - double(q[2])) - 1.0);
     }
-    const int c0 = q[0] / f0;
-    // the cell decided above, then (a point within rounding of a theta or phi edge, which the
-    // plan's atan2 and floor may have put across it) the neighbours across the nearer edges;
-    // phi wraps around
+    const int c0 = q[0] / f0, s0 = q[0] - c0 * f0;
+    {
+        const int c1 = q[1] / f1, c2 = q[2] / f2;
+        const CellRef cr = ctab[B.table_off + c0 + B.nc[0] * (c1 + B.nc[1] * c2)];
+        const int sub = s0 + f0 * ((q[1] - c1 * f1) + f1 * (q[2] - c2 * f2));
+        if ((cr.mask >> sub) & 1u)
+        {
+            out.fine = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
+            return true;
+        }
+    }
+    // a point within rounding of a theta or phi edge which the plan's atan2 and floor put in
+    // the neighbouring cell: the neighbours across the nearer edges (phi wraps around)
     const int d1 = out.xi[1] > 0.f ? 1 : -1, d2 = out.xi[2] > 0.f ? 1 : -1;
 #pragma unroll 1
-    for (int m = 0; m < 4; m++)
+    for (int m = 1; m < 4; m++)
     {
         const int e1 = (m & 1) ? d1 : 0, e2 = (m & 2) ? d2 : 0;
         const int p1 = q[1] + e1;
@@ -298,9 +343,8 @@ __device__ __forceinline__ bool locate_fast(const BoxGeo& B, const double* eth, 
         if (p1 < 0 || p1 >= n1)
             continue;
         const int c1 = p1 / f1, c2 = p2 / f2;
-        const int cell = c0 + B.nc[0] * (c1 + B.nc[1] * c2);
-        const CellRef cr = ctab[B.table_off + cell];
-        const int sub = (q[0] - c0 * f0) + f0 * ((p1 - c1 * f1) + f1 * (p2 - c2 * f2));
+        const CellRef cr = ctab[B.table_off + c0 + B.nc[0] * (c1 + B.nc[1] * c2)];
+        const int sub = s0 + f0 * ((p1 - c1 * f1) + f1 * (p2 - c2 * f2));
         if ((cr.mask >> sub) & 1u)
         {
             out.fine = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));

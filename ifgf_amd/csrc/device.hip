@@ -645,18 +645,20 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
     auto body = [&](const BoxGeo& B)
     {
         int fine;
-        double r;
+        double r, rinv;
         R x0, x1, x2;
         if constexpr (FAST)
         {
             LocF L;
             const double* eth = sang + (2 * B.fth_off - ang_base);
             const double* eph = sang + (2 * B.fph_off - ang_base);
-            if (!locate_fast(B, eth, eph, ctab, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2], kp.f0,
-                             kp.f1, kp.f2, L))
+            constexpr int FF = O0 > 0 ? 3 : 0; // the compiled order classes use factor 3
+            if (!locate_fast<FF>(B, eth, eph, ctab, y0 - B.ctr[0], y1 - B.ctr[1], y2 - B.ctr[2],
+                                 kp.f0, kp.f1, kp.f2, L))
                 return;
             fine = L.fine;
             r = L.r;
+            rinv = L.rinv;
             x0 = L.xi[0];
             x1 = L.xi[1];
             x2 = L.xi[2];
@@ -669,6 +671,7 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
                 return;
             fine = L.fine;
             r = L.r;
+            rinv = 1.0 / L.r;
             x0 = R(L.xi[0]);
             x1 = R(L.xi[1]);
             x2 = R(L.xi[2]);
@@ -676,9 +679,9 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
         const Cx<R> val = fine_eval<O0, O1, O2, R>(F, fine, kp, x0, x1, x2);
         Cx<R> cf;
         if constexpr (!PH)
-            cf = {R(INV_4PI_D / r), R(0)};
+            cf = {R(INV_4PI_D * rinv), R(0)};
         else
-            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, r), R(INV_4PI_D / r));
+            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, r), R(INV_4PI_D * rinv));
         acc = cadd(acc, cmul(val, cf));
     };
     if (ngrp == 1)
