@@ -46,6 +46,21 @@ def direct_sum_torch(x, w, idx, kappa, dev):
     return torch.cat(out).cpu().numpy()
 
 
+KERNEL_OF_PHASE = {"near": "k_near", "leaf": "k_leaf", "upward": "k_up_cell",
+                   "transform": "k_transform_t", "far": "k_far"}
+COUNTERS = os.path.join(HERE, "profiles", "counters_helmholtz_sphere_1M_kappa100_order6_mixed.json")
+
+
+def load_counters():
+    """HBM bytes per mult and launches per mult of each kernel at the default workload, from
+    the committed rocprofv3 counter summary (None when absent)."""
+    try:
+        with open(COUNTERS) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def flops_per_mult(work):
     """Algorithmic flops of one mult (transcendentals count 1 each)."""
     n_lo = work["n_lo"]
@@ -105,7 +120,7 @@ def main():
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--samples", type=int, default=1000)
-    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-stride", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -146,13 +161,18 @@ def main():
     ms, u = time_mults(op, w_dev, args.steps, args.warmup)
     print(f"# mixed: {ms:.3f} ms/matvec", file=sys.stderr, flush=True)
 
-    # per-phase device times (instrumented pass, outside the timed region)
+    # per-phase device times: the timed steps repeated with the library's phase events (HIP
+    # events on the mult's stream around every launch group, read once per mult)
     os.environ["IFGF_PHASE_TIMING"] = "1"
-    op.mult(w_dev)
+    acc = None
+    for _ in range(args.steps):
+        op.mult(w_dev)
+        t = op.last_timings()
+        acc = dict(t) if acc is None else {k: acc[k] + t[k] for k in acc}
     torch.cuda.synchronize()
-    phases = op.last_timings()
     del os.environ["IFGF_PHASE_TIMING"]
-    print(f"# phases (ms): {phases}", file=sys.stderr, flush=True)
+    phases = {k: float(v) / args.steps for k, v in acc.items()}
+    print(f"# phases (ms, mean of {args.steps}): {phases}", file=sys.stderr, flush=True)
 
     ref = direct_sum_torch(x, w, idx, args.kappa, dev)
     u_np = u.cpu().numpy()
@@ -176,21 +196,31 @@ def main():
         torch.cuda.empty_cache()
 
     fl = flops_per_mult(work)
+    default_workload = (N, args.kappa, args.order, args.leaf) == (1048576, 100.0, 6, 64)
+    counters = load_counters() if default_workload else None
     names = {"near": "near", "leaf": "leaf", "upward": "upward", "transform": "transform",
              "far": "far"}
     dom = max(names, key=lambda k: phases.get(k, 0.0))
     t_dom = phases[dom] * 1e-3
     achieved = fl[dom] / t_dom / 1e12 if t_dom > 0 else 0.0
+    traffic = None
+    kc = counters.get("kernels", {}).get(KERNEL_OF_PHASE[dom]) if counters else None
+    if kc:
+        traffic = (kc["fetch_bytes_per_mult"] + kc["write_bytes_per_mult"]) / kc["launches_per_mult"]
     roofline = {
         "bound": "mfma",
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": None,
+        "traffic": traffic,
         "kernel": dom,
-        "note": "compute-bound on the FP32 vector ALUs (MI355X FP32 vector peak = FP32 matrix "
-                "peak = 157.3 TFLOP/s); algorithmic flops per mult / the phase's device time",
+        "launches_per_mult": kc["launches_per_mult"] if kc else None,
+        "note": "compute-bound on the vector ALUs (MI355X FP32 vector peak = FP32 matrix peak = "
+                "157.3 TFLOP/s): algorithmic flops of the phase per mult / its device time "
+                "(= per launch / mean launch duration); traffic = HBM bytes per launch from "
+                "rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a plain copy (" +
+                (counters.get("source", "") if counters else "no counter profile") + ")",
         "phase_ms": phases,
         "phase_tflops": {k: (fl[k] / (phases[k] * 1e-3) / 1e12 if phases.get(k, 0) > 0 else None)
                          for k in fl},

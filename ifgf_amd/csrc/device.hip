@@ -836,6 +836,58 @@ void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t 
         c0, c1, cpb, xsz, ysz, D.cones, V, F, W, D.woff[1], D.woff[2], kp);
 }
 
+// phase timing: events recorded around each segment on the mult's stream, read after the
+// last segment (one synchronisation per mult); phases near (with gather and zeroing), leaf,
+// upward, transform, far, and the whole mult
+struct PhaseTimer
+{
+    bool on;
+    hipStream_t st;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> segs;
+    hipEvent_t open = nullptr;
+    int open_phase = -1;
+    void begin(int phase)
+    {
+        if (!on)
+            return;
+        HIP_OK(hipEventCreate(&open));
+        HIP_OK(hipEventRecord(open, st));
+        open_phase = phase;
+    }
+    void end()
+    {
+        if (!on)
+            return;
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        HIP_OK(hipEventRecord(e, st));
+        segs.push_back({open_phase, {open, e}});
+    }
+    void finish(Timing& out)
+    {
+        if (!on || segs.empty())
+            return;
+        HIP_OK(hipEventSynchronize(segs.back().second.second));
+        for (float& m : out.ms)
+            m = 0;
+        for (auto& sg : segs)
+        {
+            float x = 0;
+            HIP_OK(hipEventElapsedTime(&x, sg.second.first, sg.second.second));
+            out.ms[sg.first] += x;
+        }
+        float tot = 0;
+        HIP_OK(hipEventElapsedTime(&tot, segs.front().second.first, segs.back().second.second));
+        out.ms[5] = tot;
+        for (auto& sg : segs)
+        {
+            (void)hipEventDestroy(sg.second.first);
+            (void)hipEventDestroy(sg.second.second);
+        }
+        segs.clear();
+    }
+};
+
 template <int KIND, typename R>
 void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed,
               int io)
@@ -847,19 +899,11 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
-    hipEvent_t ev[6];
-    if (timed)
-        for (auto& e : ev)
-            HIP_OK(hipEventCreate(&e));
-    auto mark = [&](int i)
-    {
-        if (timed)
-            HIP_OK(hipEventRecord(ev[i], st));
-    };
-    mark(0);
+    PhaseTimer pt{timed, st, {}};
     const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
     const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
     const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
+    pt.begin(0);
     k_gather<R><<<dim3(unsigned((D.nsrc + 255) / 256)), dim3(256), 0, st>>>(w_dev, D.src_perm, ws,
                                                                            D.nsrc, in_im);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
@@ -867,76 +911,44 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
         k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
             D.near_items, D.n_near_items, D.near_off, D.near_src, D.src_begin, D.src_end,
             D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
-    mark(1);
+    pt.end();
+    pt.begin(1);
     if (D.n_leaf_cones)
         k_leaf<KIND, R><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(
             D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
             D.normals, ws, tabs, V, kp);
-    mark(2);
-    float t_up = 0, t_tr = 0;
+    pt.end();
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
-        if (u1 == u0 && c1 == c0)
-            continue;
-        hipEvent_t a = nullptr, b = nullptr, c = nullptr;
-        if (timed)
-        {
-            HIP_OK(hipEventCreate(&a));
-            HIP_OK(hipEventCreate(&b));
-            HIP_OK(hipEventCreate(&c));
-            HIP_OK(hipEventRecord(a, st));
-        }
         if (u1 > u0)
-            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, F, V, res);
-        if (timed)
-            HIP_OK(hipEventRecord(b, st));
-        if (c1 > c0)
-            launch_transform<R>(D, st, kp, c0, c1, V, F);
-        if (timed)
         {
-            HIP_OK(hipEventRecord(c, st));
-            HIP_OK(hipEventSynchronize(c));
-            float x = 0, y = 0;
-            HIP_OK(hipEventElapsedTime(&x, a, b));
-            HIP_OK(hipEventElapsedTime(&y, b, c));
-            t_up += x;
-            t_tr += y;
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            (void)hipEventDestroy(c);
+            pt.begin(2);
+            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, F, V, res);
+            pt.end();
+        }
+        if (c1 > c0)
+        {
+            pt.begin(3);
+            launch_transform<R>(D, st, kp, c0, c1, V, F);
+            pt.end();
         }
     }
-    mark(3);
+    pt.begin(4);
     for (size_t l = 0; l + 1 < D.far_item_off.size(); l++)
     {
         const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
         if (b > a)
             launch_up_far<PH, R>(D, st, kp, tabs, -1, int(l), a, b, F, V, res);
     }
-    mark(4);
+    pt.end();
+    pt.begin(0);
     k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(
         res, D.tgt_perm, out_dev, D.ntgt, out_re, out_im);
-    mark(5);
+    pt.end();
     HIP_OK(hipGetLastError());
-    if (timed)
-    {
-        HIP_OK(hipEventSynchronize(ev[5]));
-        float x;
-        HIP_OK(hipEventElapsedTime(&x, ev[0], ev[1]));
-        g_last.ms[0] = x;
-        HIP_OK(hipEventElapsedTime(&x, ev[1], ev[2]));
-        g_last.ms[1] = x;
-        g_last.ms[2] = t_up;
-        g_last.ms[3] = t_tr;
-        HIP_OK(hipEventElapsedTime(&x, ev[3], ev[4]));
-        g_last.ms[4] = x;
-        HIP_OK(hipEventElapsedTime(&x, ev[0], ev[5]));
-        g_last.ms[5] = x;
-        for (int i = 0; i < 6; i++)
-            (void)hipEventDestroy(ev[i]);
-    }
+    pt.finish(g_last);
 }
 
 template <typename R>
