@@ -188,6 +188,18 @@ class Oracle:
                               _p(np.ascontiguousarray(y, np.float64)), src_idx, _p(out))
         return complex(out[0], out[1])
 
+    def direct_sum(self, srcs, tgts, w, idx, normals=None, nthreads=None):
+        """FP64 direct sum of this kernel at tgts[idx] over srcs (no init needed; self term 0)."""
+        s = _pts(srcs)
+        t = _pts(tgts)
+        n = None if normals is None else _pts(normals)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+        out = np.zeros(idx.size, np.complex128)
+        lib().orc_direct(self.h, _p(s), s.shape[0], _p(w), None if n is None else _p(n), _p(t),
+                         _p(idx), idx.size, _p(out), nthreads or default_threads())
+        return out
+
     def direct(self, w, idx, nthreads=None):
         """FP64 direct sum at targets[idx] over all sources (self term 0)."""
         w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))

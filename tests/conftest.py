@@ -57,6 +57,25 @@ def perturbed_sphere(n, seed, lmax=8, amp=0.1):
     return p * r[:, None]
 
 
+def torus(n, seed, R=1.0, r=0.4):
+    """Points area-uniform on the torus (R, r) (rejection on the tube angle) and the analytic
+    unit normals."""
+    rng = np.random.default_rng(seed)
+    us, vs = [], []
+    while sum(len(u) for u in us) < n:
+        u = rng.uniform(0, 2 * np.pi, 2 * n)
+        v = rng.uniform(0, 2 * np.pi, 2 * n)
+        keep = rng.random(2 * n) < (R + r * np.cos(v)) / (R + r)
+        us.append(u[keep])
+        vs.append(v[keep])
+    u = np.concatenate(us)[:n]
+    v = np.concatenate(vs)[:n]
+    x = np.stack([(R + r * np.cos(v)) * np.cos(u), (R + r * np.cos(v)) * np.sin(u),
+                  r * np.sin(v)], axis=1)
+    nrm = np.stack([np.cos(v) * np.cos(u), np.cos(v) * np.sin(u), np.sin(v)], axis=1)
+    return x, nrm
+
+
 def cdens(n, seed):
     r = np.random.default_rng(seed)
     return r.standard_normal(n) + 1j * r.standard_normal(n)

@@ -43,3 +43,61 @@ def test_full_size_properties(gpu):
     # order 6 leaves the method's own interpolation error at ~1e-3 (measured with the oracle
     # at the same points per wavelength); FP32 arithmetic adds ~1e-6 on top
     assert err < 3e-3 and abs(err - err64) < 1e-4
+
+
+def test_combined_field_torus_mixed_vs_fp64(gpu):
+    """Combined field on a torus (R=1, r=0.4, analytic normals), order 8, tolerance 1e-5 (the
+    estimateRefinement path), N = 262,144, kappa = 64: mixed vs FP64 form and both vs an FP64
+    direct sum on 200 sampled targets."""
+    from conftest import torus
+    from oracle import pyoracle as po
+    N, kappa = 262144, 64.0
+    x, n = torus(N, 1015)
+    w = cdens(N, 2015)
+    u = {}
+    for prec in ("mixed", "fp64"):
+        op = ifgf_amd.CombinedFieldHelmholtzIfgfOperator(kappa, 64, 8, 1, 1e-5, precision=prec)
+        op.init(x.T, x.T, n.T)
+        print(f"torus {prec}: {op.plan_orders()} {op.plan_counts()}", flush=True)
+        u[prec] = op.mult(w)
+        del op
+    idx = np.random.default_rng(3015).choice(N, 200, replace=False)
+    ref = po.Oracle(po.COMBINED, kappa, 64, 8, 1, 1e-5).direct_sum(x, x, w, idx, normals=n)
+    e_mix = np.linalg.norm(u["mixed"][idx] - ref) / np.linalg.norm(ref)
+    e_64 = np.linalg.norm(u["fp64"][idx] - ref) / np.linalg.norm(ref)
+    d = np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"])
+    print(f"torus: vs direct mixed {e_mix:.2e} fp64 {e_64:.2e}; mixed vs fp64 {d:.2e}", flush=True)
+    assert d < 1e-4 and e_64 < 1e-3 and e_mix < 1e-3
+
+
+@pytest.mark.timeout(1200)
+def test_configuration5_torus_4M(gpu):
+    """N = 4,194,304 on the torus, combined field, kappa = 200, order 8, tolerance 1e-5, mixed
+    form (the orders the tolerance path picks, (8,8,8)/(5,5,5), give 12 M coarse and 169 M fine
+    cones: 218 GB of values and coefficients in the mixed form; the FP64 form would need twice
+    that, beyond one GPU's HBM): error against an FP64 direct sum on 200 sampled targets."""
+    import torch
+    from conftest import torus
+    from oracle import pyoracle as po
+    N, kappa = 4194304, 200.0
+    x, n = torus(N, 1005)
+    w = cdens(N, 2005)
+    print("configuration 5: plan", flush=True)
+    op = ifgf_amd.CombinedFieldHelmholtzIfgfOperator(kappa, 64, 8, 1, 1e-5)
+    op.init(x.T, x.T, n.T)
+    c = op.plan_counts()
+    print(f"configuration 5: {op.plan_orders()} {c}", flush=True)
+    wd = torch.as_tensor(w, device="cuda:0")
+    op.mult(wd)
+    torch.cuda.synchronize()
+    import time
+    t0 = time.time()
+    u = op.mult(wd)
+    torch.cuda.synchronize()
+    print(f"configuration 5: mult {1e3 * (time.time() - t0):.0f} ms", flush=True)
+    u = u.cpu().numpy()
+    idx = np.random.default_rng(3005).choice(N, 200, replace=False)
+    ref = po.Oracle(po.COMBINED, kappa, 64, 8, 1, 1e-5).direct_sum(x, x, w, idx, normals=n)
+    err = np.linalg.norm(u[idx] - ref) / np.linalg.norm(ref)
+    print(f"configuration 5: rel. error vs direct sum {err:.2e}", flush=True)
+    assert err < 1e-3
