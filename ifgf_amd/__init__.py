@@ -10,8 +10,10 @@ from .pyifgf import (  # noqa: F401
     GradHelmholtzIfgfOperator,
     HelmholtzIfgfOperator,
     LaplaceIfgfOperator,
+    MergedIfgfOperator,
     MofifiedHelmholtzIfgfOperator,
     ModifiedHelmholtzIfgfOperator,
+    merge,
 )
 
 __all__ = [
@@ -23,4 +25,6 @@ __all__ = [
     "LaplaceIfgfOperator",
     "DoubleLayerHelmholtzIfgfOperator",
     "CombinedFieldHelmholtzIfgfOperator",
+    "MergedIfgfOperator",
+    "merge",
 ]

@@ -31,6 +31,8 @@ SIGNATURES = {
     "ifgf_init": (_I, [_P, _P, _I64, _P, _I64, _P, CUTOFF_FN, _P, _I]),
     "ifgf_mult": (_I, [_P, _P, _I64, _P, _I, _P]),
     "ifgf_mult_adjoint": (_I, [_P, _P, _I64, _P, _I, _P]),
+    "ifgf_merge": (_I, [_P, _I, _I, ctypes.POINTER(_P)]),
+    "ifgf_sizes": (_I, [_P, _P, _P]),
     "ifgf_destroy": (None, [_P]),
     "ifgf_last_error": (ctypes.c_char_p, [_P]),
     "ifgf_version": (ctypes.c_char_p, []),

@@ -89,6 +89,7 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
                                                int64_t nitems,
                                                const int32_t* __restrict__ near_off,
                                                const int32_t* __restrict__ near_src,
+                                               const BoxGeo* __restrict__ geo,
                                                const int64_t* __restrict__ sbeg,
                                                const int64_t* __restrict__ send,
                                                const double* __restrict__ spts,
@@ -120,6 +121,7 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
     for (int32_t j = j0; j < j1; j++)
     {
         const int32_t s = near_src[j];
+        const KParams kb = with_k(kp, geo[s]);
         const int64_t b = sbeg[s], e = send[s];
         for (int64_t c0 = b; c0 < e; c0 += 64)
         {
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
                 const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
                         x2 = R(sp[3 * k + 2] - y2);
                 const Cx<R> g =
-                    p2p<KIND, R>(kp, x0, x1, x2, NRM ? sn + 3 * k : nullptr, Cx<R>{1, 0});
+                    p2p<KIND, R>(kb, x0, x1, x2, NRM ? sn + 3 * k : nullptr, Cx<R>{1, 0});
 #pragma unroll
                 for (int r = 0; r < RB; r++)
                     acc[r] = cadd(acc[r], cmul(g, sw[k * RB + r]));
@@ -192,6 +194,7 @@ __global__ __launch_bounds__(128) void k_leaf_b(const int32_t* __restrict__ work
     const int32_t g = work[wi];
     const ConeDev cn = cones[g];
     const BoxGeo B = geo[cn.box];
+    const KParams kb = with_k(kp, B);
     const int64_t b0 = sbeg[cn.box], b1 = send[cn.box];
     for (int q0 = 0; q0 < kp.n_hi; q0 += 128)
     {
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(128) void k_leaf_b(const int32_t* __restrict__ work
                         nn[1] = ny[k];
                         nn[2] = nz[k];
                     }
-                    const Cx<R> f = factored<KIND, R>(kp, y0, y1, y2, dc, sx[k], sy[k], sz[k],
+                    const Cx<R> f = factored<KIND, R>(kb, y0, y1, y2, dc, sx[k], sy[k], sz[k],
                                                       nn, Cx<R>{1, 0});
 #pragma unroll
                     for (int r = 0; r < RB; r++)
@@ -264,6 +267,7 @@ __global__ __launch_bounds__(256) void k_up_cell_b(const WorkItem* __restrict__ 
                                                    const int32_t* __restrict__ child,
                                                    const CellRef* __restrict__ ctab,
                                                    const UpEnt<R>* __restrict__ ents,
+                                                   const int64_t* __restrict__ part_ent,
                                                    const Cx<R>* __restrict__ F,
                                                    Cx<R>* __restrict__ V, KParams kp)
 {
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(256) void k_up_cell_b(const WorkItem* __restrict__ 
     const int r = tid % RB;
     for (int q = tid / RB; q < kp.n_hi; q += 256 / RB)
     {
-        const UpEnt<R>* e = ents + (int64_t(wi.box) * kp.n_hi + q) * 8;
+        const UpEnt<R>* e = ents + part_ent[wi.pad] + (int64_t(wi.box) * kp.n_hi + q) * 8;
         Cx<R> acc[K];
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -368,7 +372,7 @@ __global__ __launch_bounds__(256) void k_up_b(const int32_t* __restrict__ work, 
             if constexpr (!PH)
                 tf = {R(rp / L.r), R(0)};
             else
-                tf = cscale(expmk<R>(kp.k_re, kp.k_im, R(L.r - rp)), R(rp / L.r));
+                tf = cscale(expmk<R>(P.k_re, P.k_im, R(L.r - rp)), R(rp / L.r));
             acc = cadd(acc, cmul(val, tf));
         }
         V[(int64_t(g) * kp.n_hi + q) * RB + r] = acc;
@@ -589,7 +593,7 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
                     if constexpr (!PH)
                         cf = {R(INV_4PI_D * ri), R(0)};
                     else
-                        cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, rr), R(INV_4PI_D * ri));
+                        cf = cscale(expmk_far<R>(B.k_re, B.k_im, rr), R(INV_4PI_D * ri));
                 }
                 else
                     slot = -1;
@@ -686,7 +690,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
     k_zero_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(res, nt);
     if (D.n_near_items)
         k_near_b<KIND, R, RB><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
-            D.near_items, D.n_near_items, D.near_off, D.near_src, D.src_begin, D.src_end,
+            D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin, D.src_end,
             D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
     if (D.n_leaf_cones)
         k_leaf_b<KIND, R, RB><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(
@@ -696,21 +700,14 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
+        const int64_t i0 = D.up_item_off[l], i1 = D.up_item_off[l + 1];
+        if (i1 > i0)
+            k_up_cell_b<R, O0, O1, O2, UP_K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(
+                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[l], D.cones, D.geo, D.child,
+                D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp);
         if (u1 > u0)
-        {
-            if (D.up_tab[l] >= 0)
-            {
-                const int64_t i0 = D.up_item_off[l], i1 = D.up_item_off[l + 1];
-                k_up_cell_b<R, O0, O1, O2, UP_K, RB>
-                    <<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(
-                        D.up_items + i0, i1 - i0, D.up_cones + u0, D.cones, D.geo, D.child,
-                        D.ctab, static_cast<const UpEnt<R>*>(D.up_entries) + D.up_ent_off[l], F,
-                        V, kp);
-            }
-            else
-                k_up_b<PH, R, O0, O1, O2, RB><<<dim3(unsigned(u1 - u0)), dim3(256), 0, st>>>(
-                    D.up_cones + u0, u1 - u0, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
-        }
+            k_up_b<PH, R, O0, O1, O2, RB><<<dim3(unsigned(u1 - u0)), dim3(256), 0, st>>>(
+                D.up_cones + u0, u1 - u0, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
         if (c1 > c0)
         {
             if constexpr (P6)

@@ -28,6 +28,7 @@ struct ifgf_operator
     // adjoint products: same_points (targets are the sources, in order) reuses this plan;
     // otherwise a transposed operator (sources and targets swapped) is built on first use
     bool same_points = false;
+    bool merged = false; // several geometries (ifgf_merge)
     ifgf_cutoff_fn cutoff = nullptr;
     void* cutoff_ctx = nullptr;
     std::unique_ptr<ifgf_operator> adj;
@@ -172,6 +173,9 @@ extern "C"
         try
         {
             ifgf_operator* B = op;
+            if (!op->same_points && op->merged)
+                return fail(op, IFGF_EINVAL,
+                            "the adjoint of merged geometries needs targets = sources in each");
             if (!op->same_points)
             {
                 if (!op->adj)
@@ -226,6 +230,74 @@ extern "C"
         return IFGF_OK;
     }
 
+    int ifgf_merge(ifgf_operator* const* ops, int n_ops, int flags, ifgf_operator** out)
+    {
+        if (!out)
+            return IFGF_EINVAL;
+        *out = nullptr;
+        if (!ops || n_ops < 1)
+            return IFGF_EINVAL;
+        std::vector<const Plan*> parts;
+        bool same = true;
+        for (int i = 0; i < n_ops; i++)
+        {
+            if (!ops[i] || !ops[i]->planned || ops[i]->merged)
+                return IFGF_ESTATE;
+            parts.push_back(&ops[i]->plan);
+            same = same && ops[i]->same_points;
+        }
+        const bool plan_only = (flags & IFGF_INIT_PLAN_ONLY) != 0;
+        if (!plan_only && !device_available())
+            return IFGF_ENODEV;
+        auto m = std::make_unique<ifgf_operator>();
+        const ifgf_operator& o0 = *ops[0];
+        m->K = o0.K;
+        m->leaf = o0.leaf;
+        m->order = o0.order;
+        m->n_elem = o0.n_elem;
+        m->precision = o0.precision;
+        m->merged = true;
+        m->same_points = same;
+        try
+        {
+            std::string e = merge_plans(parts, m->plan);
+            if (!e.empty())
+            {
+                ops[0]->err = e;
+                return IFGF_EINVAL;
+            }
+            m->planned = true;
+            if (!plan_only)
+            {
+                e = device_upload(m->plan, m->precision, m->dev);
+                if (!e.empty())
+                {
+                    ops[0]->err = e;
+                    return IFGF_EINTERNAL;
+                }
+                m->on_device = true;
+            }
+        }
+        catch (const std::exception& ex)
+        {
+            ops[0]->err = ex.what();
+            return IFGF_EINTERNAL;
+        }
+        *out = m.release();
+        return IFGF_OK;
+    }
+
+    int ifgf_sizes(const ifgf_operator* op, int64_t* n_src, int64_t* n_tgt)
+    {
+        if (!op || !n_src || !n_tgt)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return IFGF_ESTATE;
+        *n_src = op->plan.S.n;
+        *n_tgt = op->plan.T.n;
+        return IFGF_OK;
+    }
+
     void ifgf_destroy(ifgf_operator* op)
     {
         if (!op)
@@ -265,6 +337,11 @@ extern "C"
                 c[12]++;
                 c[13] += int64_t(U.entries.size());
             }
+        for (const auto& pp : P.up_parts)
+        {
+            c[12]++;
+            c[13] += int64_t(pp.U.entries.size());
+        }
         return IFGF_OK;
     }
 
@@ -351,7 +428,12 @@ extern "C"
         if (mask)
             std::memcpy(mask, P.cone_mask.data(), P.cone_mask.size() * sizeof(uint32_t));
         if (fine_off)
+        {
+            if (int64_t(P.fine_off.size()) != nb + 1)
+                return fail(const_cast<ifgf_operator*>(op), IFGF_EINTERNAL,
+                            "plan has no fine cone lists");
             std::memcpy(fine_off, P.fine_off.data(), (nb + 1) * sizeof(int64_t));
+        }
         if (fine_ids)
             std::memcpy(fine_ids, P.fine_ids.data(), P.fine_ids.size() * sizeof(int64_t));
         return IFGF_OK;

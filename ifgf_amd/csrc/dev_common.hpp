@@ -737,6 +737,16 @@ struct UpEntD
 };
 template <typename R> using UpEnt = typename std::conditional<sizeof(R) == 4, UpEntF, UpEntD>::type;
 
+// the kernel parameters with the wavenumber of box B (plans of several geometries carry one
+// per box)
+__device__ __forceinline__ KParams with_k(KParams kp, const BoxGeo& B)
+{
+    kp.k_re = B.k_re;
+    kp.k_im = B.k_im;
+    kp.kappa = B.kappa;
+    return kp;
+}
+
 inline KParams kparams(const DevicePlan& D)
 {
     KParams k;

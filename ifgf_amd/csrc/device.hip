@@ -72,6 +72,7 @@ template <int KIND, typename R>
 __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items, int64_t nitems,
                                              const int32_t* __restrict__ near_off,
                                              const int32_t* __restrict__ near_src,
+                                             const BoxGeo* __restrict__ geo,
                                              const int64_t* __restrict__ sbeg,
                                              const int64_t* __restrict__ send,
                                              const double* __restrict__ spts,
@@ -100,6 +101,7 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
     for (int32_t j = j0; j < j1; j++)
     {
         const int32_t s = near_src[j];
+        const KParams kb = with_k(kp, geo[s]);
         const int64_t b = sbeg[s], e = send[s];
         for (int64_t c0 = b; c0 < e; c0 += 64)
         {
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
             {
                 const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
                         x2 = R(sp[3 * k + 2] - y2);
-                acc = cadd(acc, p2p<KIND, R>(kp, x0, x1, x2, NRM ? sn + 3 * k : nullptr, sw[k]));
+                acc = cadd(acc, p2p<KIND, R>(kb, x0, x1, x2, NRM ? sn + 3 * k : nullptr, sw[k]));
             }
         }
     }
@@ -160,6 +162,7 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
     const int32_t g = work[wi];
     const ConeDev cn = cones[g];
     const BoxGeo B = geo[cn.box];
+    const KParams kb = with_k(kp, B);
     const int64_t b0 = sbeg[cn.box], b1 = send[cn.box];
     for (int q0 = 0; q0 < kp.n_hi; q0 += CH)
     {
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
                         nn[1] = ny[k];
                         nn[2] = nz[k];
                     }
-                    acc = cadd(acc, factored<KIND, R>(kp, y0, y1, y2, dc, sx[k], sy[k], sz[k], nn,
+                    acc = cadd(acc, factored<KIND, R>(kb, y0, y1, y2, dc, sx[k], sy[k], sz[k], nn,
                                                       sw[k]));
                 }
         }
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
             if constexpr (!PH)
                 tf = {R(rp / L.r), R(0)};
             else
-                tf = cscale(expmk<R>(kp.k_re, kp.k_im, R(L.r - rp)), R(rp / L.r));
+                tf = cscale(expmk<R>(P.k_re, P.k_im, R(L.r - rp)), R(rp / L.r));
             acc = cadd(acc, cmul(val, tf));
         }
         V[int64_t(g) * kp.n_hi + q] = acc;
@@ -255,61 +258,12 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
 }
 
 
-// upward pass with the precomputed geometry: per (node, child) one table entry gives the
-// child's fine cell, local coordinates and transfer factor; only the coefficient lookup and
-// the Clenshaw sum remain (ifgfoperator.hpp:631-688, 1639-1705)
-template <typename R, int O0, int O1, int O2>
-__global__ __launch_bounds__(128) void k_up_tab(const int32_t* __restrict__ work, int64_t nwork,
-                                                const ConeDev* __restrict__ cones,
-                                                const BoxGeo* __restrict__ geo,
-                                                const int32_t* __restrict__ child,
-                                                const CellRef* __restrict__ ctab,
-                                                const int32_t* __restrict__ cellmap,
-                                                const UpEnt<R>* __restrict__ ents,
-                                                const Cx<R>* __restrict__ F, Cx<R>* __restrict__ V,
-                                                KParams kp)
-{
-    const int64_t wi = xcd_block();
-    if (wi >= nwork)
-        return;
-    const int32_t g = work[wi];
-    const ConeDev cn = cones[g];
-    const int64_t cu = cellmap[cn.cell];
-    int32_t ch[8];
-    int64_t toff[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++)
-    {
-        ch[c] = child[8 * cn.box + c];
-        toff[c] = ch[c] >= 0 ? geo[ch[c]].table_off : 0;
-    }
-    for (int q = threadIdx.x; q < kp.n_hi; q += blockDim.x)
-    {
-        const UpEnt<R>* e = ents + (cu * kp.n_hi + q) * 8;
-        Cx<R> acc = {0, 0};
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-        {
-            if (ch[c] < 0)
-                continue;
-            const UpEnt<R> E = e[c];
-            if (E.cf < 0)
-                continue;
-            const CellRef cr = ctab[toff[c] + E.cf];
-            if (!((cr.mask >> E.sub) & 1u))
-                continue;
-            const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
-            const Cx<R> val = fine_eval<O0, O1, O2, R>(F, slot, kp, E.xi0, E.xi1, E.xi2);
-            acc = cadd(acc, cmul(val, Cx<R>{E.tr, E.ti}));
-        }
-        V[int64_t(g) * kp.n_hi + q] = acc;
-    }
-}
-
-// The same with the parent cones grouped by cell: a workgroup takes up to K parent cones of
-// one cell (item.box = the cell's index in the level's entries, glist[item.first ..] the
-// cones), so each table entry is read once for K boxes.  Children are summed in order for
-// every cone, as in k_up_tab.
+// upward pass with the precomputed geometry (per parent level of a geometry, see UpLevel in
+// plan.hpp): per (node, child) one table entry gives the child's fine cell, local coordinates
+// and transfer factor.  A workgroup takes up to K parent cones of one cell (item.box = the
+// cell's index in its table, item.pad = the table part, glist[item.first ..] the cones), so
+// each entry is read once for K boxes; children are summed in order for every cone
+// (ifgfoperator.hpp:631-688, 1639-1705).
 template <typename R, int O0, int O1, int O2, int K>
 __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ items,
                                                  int64_t nitems,
@@ -319,6 +273,7 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
                                                  const int32_t* __restrict__ child,
                                                  const CellRef* __restrict__ ctab,
                                                  const UpEnt<R>* __restrict__ ents,
+                                                 const int64_t* __restrict__ part_ent,
                                                  const Cx<R>* __restrict__ F,
                                                  Cx<R>* __restrict__ V, KParams kp)
 {
@@ -348,7 +303,7 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
     const int n = wi.count;
     for (int q = tid; q < kp.n_hi; q += blockDim.x)
     {
-        const UpEnt<R>* e = ents + (int64_t(wi.box) * kp.n_hi + q) * 8;
+        const UpEnt<R>* e = ents + part_ent[wi.pad] + (int64_t(wi.box) * kp.n_hi + q) * 8;
         Cx<R> acc[K];
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -681,7 +636,7 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
         if constexpr (!PH)
             cf = {R(INV_4PI_D * rinv), R(0)};
         else
-            cf = cscale(expmk_far<R>(kp.k_re, kp.k_im, r), R(INV_4PI_D * rinv));
+            cf = cscale(expmk_far<R>(B.k_re, B.k_im, r), R(INV_4PI_D * rinv));
         acc = cadd(acc, cmul(val, cf));
     };
     if (ngrp == 1)
@@ -752,16 +707,18 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
                      Cx<R>* res)
 {
     const bool up = up_level >= 0;
-    if (up && D.up_tab[up_level] >= 0)
+    if (up)
     {
         const int64_t i0 = D.up_item_off[up_level], i1 = D.up_item_off[up_level + 1];
-        k_up_cell<R, O0, O1, O2, UP_K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
-            D.up_items + i0, i1 - i0, D.up_cones + a, D.cones, D.geo, D.child, D.ctab,
-            static_cast<const UpEnt<R>*>(D.up_entries) + D.up_ent_off[up_level], F, V, kp);
+        if (i1 > i0)
+            k_up_cell<R, O0, O1, O2, UP_K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
+                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,
+                D.child, D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F,
+                V, kp);
+        if (b > a)
+            k_up<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
+                D.up_cones + a, b - a, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
     }
-    else if (up)
-        k_up<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
-            D.up_cones + a, b - a, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
     else
     {
         const int abase = D.far_ang_base[far_level], an = D.far_ang_n[far_level];
@@ -909,7 +866,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
     if (D.n_near_items)
         k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
-            D.near_items, D.n_near_items, D.near_off, D.near_src, D.src_begin, D.src_end,
+            D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin, D.src_end,
             D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
     pt.end();
     pt.begin(1);
@@ -922,7 +879,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
-        if (u1 > u0)
+        if (u1 > u0 || D.up_item_off[l + 1] > D.up_item_off[l])
         {
             pt.begin(2);
             launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, F, V, res);
@@ -986,7 +943,7 @@ void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
-                    D.trig_ph, D.fine_ang, D.wmat, D.up_cellmap, D.up_entries, D.up_items,
+                    D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items, D.up_glist,
                     D.leaf_cones, D.up_cones, D.far_off, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
@@ -1058,6 +1015,9 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                 B.nc[d] = int32_t(P.nc[3 * g + d]);
             }
             B.H = P.S.side[g];
+            B.k_re = P.box_k.empty() ? P.K.k.real() : P.box_k[3 * g];
+            B.k_im = P.box_k.empty() ? P.K.k.imag() : P.box_k[3 * g + 1];
+            B.kappa = P.box_k.empty() ? P.K.kappa : P.box_k[3 * g + 2];
             B.table_off = P.table_off[g];
             const auto key = std::make_pair(P.nc[3 * g + 1], P.nc[3 * g + 2]);
             auto it = cls.find(key);
@@ -1172,24 +1132,35 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                 D.wmat = dupload(Wf, tot);
             }
         }
-        // upward geometry tables
+        // upward geometry tables: one part per level with a valid table (per geometry and level
+        // in a plan of several geometries), entries concatenated; every parent box knows its
+        // part
+        std::vector<const UpLevel*> parts;
+        std::vector<int32_t> box_part(nb, -1);
+        if (P.up_parts.empty())
         {
-            std::vector<int32_t> cmap;
+            for (int l = 0; l < D.levels && l < int(P.up.size()); l++)
+                if (P.up[l].valid)
+                {
+                    for (int64_t g = P.S.level_off[l]; g < P.S.level_off[l + 1]; g++)
+                        box_part[g] = int32_t(parts.size());
+                    parts.push_back(&P.up[l]);
+                }
+        }
+        else
+        {
+            for (const auto& pp : P.up_parts)
+                parts.push_back(&pp.U);
+            box_part = P.box_up_part;
+        }
+        {
             std::vector<UpEntF> ef;
             std::vector<UpEntD> ed;
-            D.up_tab.assign(D.levels, -1);
-            D.up_map_off.assign(D.levels, 0);
-            D.up_ent_off.assign(D.levels, 0);
-            for (int l = 0; l < D.levels && l < int(P.up.size()); l++)
+            std::vector<int64_t> part_ent;
+            for (const UpLevel* U : parts)
             {
-                const UpLevel& U = P.up[l];
-                if (!U.valid)
-                    continue;
-                D.up_tab[l] = l;
-                D.up_map_off[l] = int64_t(cmap.size());
-                D.up_ent_off[l] = int64_t(precision == PREC_FP64 ? ed.size() : ef.size());
-                cmap.insert(cmap.end(), U.cellmap.begin(), U.cellmap.end());
-                for (const UpEntry& E : U.entries)
+                part_ent.push_back(int64_t(precision == PREC_FP64 ? ed.size() : ef.size()));
+                for (const UpEntry& E : U->entries)
                 {
                     if (precision == PREC_FP64)
                         ed.push_back({E.cf, E.sub, E.xi[0], E.xi[1], E.xi[2], E.tf[0], E.tf[1]});
@@ -1198,15 +1169,21 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                                       float(E.tf[0]), float(E.tf[1]), 0.f});
                 }
             }
-            D.up_cellmap = dupload(cmap, tot);
+            D.n_up_parts = int64_t(parts.size());
+            D.up_part_ent = dupload(part_ent, tot);
             if (precision == PREC_FP64)
                 D.up_entries = dupload(ed, tot);
             else
                 D.up_entries = dupload(ef, tot);
         }
-        // work lists
-        std::vector<int32_t> leaf, up;
+        // work lists.  Per level: leaf cones (leaf sampling), parent cones of boxes with a table
+        // grouped by (part, cell) into items of at most UP_K cones of one cell (boxes in order
+        // within a cell), and the other parent cones (per-box upward pass)
+        std::vector<int32_t> leaf, up, glist;
+        std::vector<WorkItem> uitems;
         D.up_off.assign(D.levels + 1, 0);
+        D.up_gl_off.assign(D.levels + 1, 0);
+        D.up_item_off.assign(D.levels + 1, 0);
         D.cone_lvl_off.assign(D.levels + 1, 0);
         for (int l = 0; l < D.levels; l++)
         {
@@ -1214,39 +1191,34 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             D.cone_lvl_off[l] = P.cone_off[b0];
             D.cone_lvl_off[l + 1] = P.cone_off[b1];
             D.up_off[l] = int64_t(up.size());
+            D.up_gl_off[l] = int64_t(glist.size());
+            D.up_item_off[l] = int64_t(uitems.size());
+            const int64_t g0 = int64_t(glist.size());
             for (int64_t g = b0; g < b1; g++)
                 for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
-                    (P.S.leaf[g] ? leaf : up).push_back(int32_t(c));
-            D.up_off[l + 1] = int64_t(up.size());
-        }
-        // levels with upward tables: parent cones grouped by cell (boxes in order within a
-        // cell), in items of at most UP_K cones of one cell
-        std::vector<WorkItem> uitems;
-        D.up_item_off.assign(D.levels + 1, 0);
-        for (int l = 0; l < D.levels; l++)
-        {
-            D.up_item_off[l] = int64_t(uitems.size());
-            const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
-            if (D.up_tab[l] < 0 || u1 == u0)
-                continue;
-            const std::vector<int32_t>& cm = P.up[l].cellmap;
-            auto cu = [&](int32_t c) { return cm[P.cone_cell[c]]; };
-            std::stable_sort(up.begin() + u0, up.begin() + u1,
-                             [&](int32_t a, int32_t b) { return cu(a) < cu(b); });
-            for (int64_t i = u0; i < u1;)
+                    (P.S.leaf[g] ? leaf : (box_part[g] >= 0 ? glist : up)).push_back(int32_t(c));
+            auto part = [&](int32_t c) { return box_part[P.cone_box[c]]; };
+            auto cu = [&](int32_t c) { return parts[part(c)]->cellmap[P.cone_cell[c]]; };
+            std::stable_sort(glist.begin() + g0, glist.end(), [&](int32_t a, int32_t b)
+                             { return part(a) != part(b) ? part(a) < part(b) : cu(a) < cu(b); });
+            for (int64_t i = g0; i < int64_t(glist.size());)
             {
-                const int32_t u = cu(up[i]);
+                const int32_t pa = part(glist[i]), u = cu(glist[i]);
                 if (u < 0)
                     throw std::runtime_error("upward table has no entry for an active cell");
                 int64_t j = i;
-                while (j < u1 && j - i < UP_K && cu(up[j]) == u)
+                while (j < int64_t(glist.size()) && j - i < UP_K && part(glist[j]) == pa &&
+                       cu(glist[j]) == u)
                     j++;
-                uitems.push_back({u, int32_t(i - u0), int32_t(j - i), 0});
+                uitems.push_back({u, int32_t(i - g0), int32_t(j - i), pa});
                 i = j;
             }
         }
+        D.up_off[D.levels] = int64_t(up.size());
+        D.up_gl_off[D.levels] = int64_t(glist.size());
         D.up_item_off[D.levels] = int64_t(uitems.size());
         D.up_items = dupload(uitems, tot);
+        D.up_glist = dupload(glist, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
         D.up_cones = dupload(up, tot);

@@ -32,6 +32,7 @@ struct BoxGeo
     int32_t ph_off;
     int32_t fth_off;  // fine-grid theta table: (cos, sin) of the nf1+1 cell edges, then centres
     int32_t fph_off;  // fine-grid phi table: the same for the nf2 cells
+    double k_re, k_im, kappa; // wavenumber of the box's geometry (e^{-k d}; kappa: real form)
 };
 
 struct ConeDev
@@ -90,19 +91,20 @@ struct DevicePlan
     void* wmat = nullptr;         // axis maps (R): W0 | W1 | W2
     int woff[4] = {0, 0, 0, 0};
 
-    // upward geometry tables (per parent level): cell -> compact index, and entries
-    // {cf, sub, xi[3], tf[2]} in R; up_tab[l] < 0 when level l uses the per-box path
-    int32_t* up_cellmap = nullptr;
+    // upward geometry tables: entries {cf, sub, xi[3], tf[2]} in R of every part (a level of
+    // one geometry), up_part_ent[part] = the part's first entry
     void* up_entries = nullptr;
-    std::vector<int64_t> up_map_off, up_ent_off;
-    std::vector<int> up_tab;
+    int64_t* up_part_ent = nullptr;
+    int64_t n_up_parts = 0;
 
-    WorkItem* up_items = nullptr;            // table levels: (cell, cones) items
+    WorkItem* up_items = nullptr;            // (cell, cones of that cell, part) items
     std::vector<int64_t> up_item_off;        // per level, into up_items
+    int32_t* up_glist = nullptr;             // the items' cones, level-major
+    std::vector<int64_t> up_gl_off;          // per level, into up_glist
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
-    int32_t* up_cones = nullptr;             // non-leaf cones, level-major
+    int32_t* up_cones = nullptr;             // non-leaf cones without a table, level-major
     std::vector<int64_t> up_off;             // per level, into up_cones
     std::vector<int64_t> cone_lvl_off;       // per level, global cone range
 

@@ -1123,4 +1123,244 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
     build_cones(P, nthreads);
     return "";
 }
+// ---------------------------------------------------------------------------------------
+// several geometries in one plan
+// ---------------------------------------------------------------------------------------
+namespace
+{
+// new ids of every part's boxes: level-major, parts in order within a level
+std::vector<std::vector<int32_t>> merged_ids(const std::vector<const Tree*>& trees, int levels,
+                                             std::vector<int64_t>& level_off)
+{
+    const size_t np = trees.size();
+    level_off.assign(levels + 1, 0);
+    std::vector<std::vector<int32_t>> ids(np);
+    int64_t next = 0;
+    for (int l = 0; l < levels; l++)
+    {
+        level_off[l] = next;
+        for (size_t g = 0; g < np; g++)
+        {
+            const Tree& T = *trees[g];
+            if (ids[g].empty())
+                ids[g].assign(T.num_boxes(), -1);
+            if (l >= T.levels)
+                continue;
+            for (int64_t b = T.level_off[l]; b < T.level_off[l + 1]; b++)
+                ids[g][b] = int32_t(next++);
+        }
+    }
+    level_off[levels] = next;
+    return ids;
+}
+
+void merge_tree(const std::vector<const Tree*>& trees, const std::vector<std::vector<int32_t>>& ids,
+                const std::vector<int64_t>& level_off, int levels, Tree& M)
+{
+    const size_t np = trees.size();
+    M = Tree();
+    M.levels = levels;
+    M.level_off = level_off;
+    const int64_t nb = level_off[levels];
+    M.begin.assign(nb, 0);
+    M.end.assign(nb, 0);
+    M.parent.assign(nb, -1);
+    M.child.assign(8 * nb, -1);
+    M.leaf.assign(nb, 0);
+    M.blo.assign(3 * nb, 0);
+    M.bhi.assign(3 * nb, 0);
+    M.ctr.assign(3 * nb, 0);
+    M.side.assign(nb, 0);
+    int64_t poff = 0;
+    for (size_t g = 0; g < np; g++)
+    {
+        const Tree& T = *trees[g];
+        for (int64_t b = 0; b < T.num_boxes(); b++)
+        {
+            const int64_t m = ids[g][b];
+            M.begin[m] = T.begin[b] + poff;
+            M.end[m] = T.end[b] + poff;
+            M.parent[m] = T.parent[b] < 0 ? -1 : ids[g][T.parent[b]];
+            for (int c = 0; c < 8; c++)
+                M.child[8 * m + c] = T.child[8 * b + c] < 0 ? -1 : ids[g][T.child[8 * b + c]];
+            M.leaf[m] = T.leaf[b];
+            for (int d = 0; d < 3; d++)
+            {
+                M.blo[3 * m + d] = T.blo[3 * b + d];
+                M.bhi[3 * m + d] = T.bhi[3 * b + d];
+                M.ctr[3 * m + d] = T.ctr[3 * b + d];
+            }
+            M.side[m] = T.side[b];
+        }
+        M.pts.insert(M.pts.end(), T.pts.begin(), T.pts.end());
+        for (int64_t i = 0; i < T.n; i++)
+            M.perm.push_back(T.perm[i] + poff);
+        poff += T.n;
+    }
+    M.n = poff;
+    for (int d = 0; d < 3; d++)
+    {
+        M.lo[d] = trees[0]->lo[d];
+        M.hi[d] = trees[0]->hi[d];
+    }
+}
+} // namespace
+
+std::string merge_plans(const std::vector<const Plan*>& parts, Plan& M)
+{
+    if (parts.empty())
+        return "no plans to merge";
+    const Plan& P0 = *parts[0];
+    for (const Plan* p : parts)
+    {
+        if (p->K.kind != P0.K.kind || p->K.dx != P0.K.dx)
+            return "merged plans need one kernel family (and one derivative direction)";
+        for (int d = 0; d < 3; d++)
+            if (p->ho_order[d] != P0.ho_order[d] || p->lo_order[d] != P0.lo_order[d] ||
+                p->factor[d] != P0.factor[d])
+                return "merged plans need the same interpolation orders";
+        if (p->normals.empty() != P0.normals.empty())
+            return "merged plans need normals in all parts or in none";
+        if (!p->up_parts.empty())
+            return "plans to merge are single geometries";
+    }
+    const size_t np = parts.size();
+    M = Plan();
+    M.K = P0.K;
+    for (int d = 0; d < 3; d++)
+    {
+        M.base_els[d] = P0.base_els[d];
+        M.base_order[d] = P0.base_order[d];
+        M.ho_order[d] = P0.ho_order[d];
+        M.lo_order[d] = P0.lo_order[d];
+        M.factor[d] = P0.factor[d];
+    }
+    int ls = 0, lt = 0;
+    std::vector<const Tree*> st, tt;
+    for (const Plan* p : parts)
+    {
+        ls = std::max(ls, p->S.levels);
+        lt = std::max(lt, p->T.levels);
+        st.push_back(&p->S);
+        tt.push_back(&p->T);
+    }
+    std::vector<int64_t> loff_s, loff_t;
+    const auto sid = merged_ids(st, ls, loff_s);
+    const auto tid = merged_ids(tt, lt, loff_t);
+    merge_tree(st, sid, loff_s, ls, M.S);
+    merge_tree(tt, tid, loff_t, lt, M.T);
+    const int64_t nb = M.S.num_boxes();
+    M.part_src_off.push_back(0);
+    M.part_tgt_off.push_back(0);
+    for (const Plan* p : parts)
+    {
+        M.part_src_off.push_back(M.part_src_off.back() + p->S.n);
+        M.part_tgt_off.push_back(M.part_tgt_off.back() + p->T.n);
+    }
+    // pairs, sorted by target box
+    for (size_t g = 0; g < np; g++)
+    {
+        for (const Pair& q : parts[g]->far)
+            M.far.push_back({sid[g][q.src], tid[g][q.tgt]});
+        for (const Pair& q : parts[g]->near)
+            M.near.push_back({sid[g][q.src], tid[g][q.tgt]});
+        M.n_tofar += parts[g]->n_tofar;
+        M.normals.insert(M.normals.end(), parts[g]->normals.begin(), parts[g]->normals.end());
+    }
+    auto by_tgt = [](const Pair& a, const Pair& b) { return a.tgt < b.tgt; };
+    std::stable_sort(M.far.begin(), M.far.end(), by_tgt);
+    std::stable_sort(M.near.begin(), M.near.end(), by_tgt);
+    // per source box data in the new order
+    std::vector<std::pair<int32_t, int64_t>> origin(nb); // new box -> (part, old box)
+    for (size_t g = 0; g < np; g++)
+        for (int64_t b = 0; b < int64_t(sid[g].size()); b++)
+            origin[sid[g][b]] = {int32_t(g), b};
+    M.has_far_anc.resize(nb);
+    M.parent_has_far.resize(nb);
+    M.dom_lo.resize(3 * nb);
+    M.dom_hi.resize(3 * nb);
+    M.nc.resize(3 * nb);
+    M.nf.resize(3 * nb);
+    M.hc.resize(3 * nb);
+    M.hf.resize(3 * nb);
+    M.box_k.resize(3 * nb);
+    M.box_up_part.assign(nb, -1);
+    M.table_off.assign(nb + 1, 0);
+    M.cone_off.assign(nb + 1, 0);
+    // up tables: one part per (geometry, valid level)
+    std::vector<std::vector<int32_t>> part_of(np);
+    for (size_t g = 0; g < np; g++)
+    {
+        part_of[g].assign(parts[g]->up.size(), -1);
+        for (size_t l = 0; l < parts[g]->up.size(); l++)
+            if (parts[g]->up[l].valid)
+            {
+                part_of[g][l] = int32_t(M.up_parts.size());
+                M.up_parts.push_back({int(l), parts[g]->up[l]});
+            }
+    }
+    std::vector<std::vector<int32_t>> cone_map(np);
+    for (size_t g = 0; g < np; g++)
+        cone_map[g].assign(parts[g]->num_cones(), -1);
+    for (int64_t m = 0; m < nb; m++)
+    {
+        const Plan& P = *parts[origin[m].first];
+        const int64_t b = origin[m].second;
+        const int g = origin[m].first;
+        M.has_far_anc[m] = P.has_far_anc[b];
+        M.parent_has_far[m] = P.parent_has_far[b];
+        for (int d = 0; d < 3; d++)
+        {
+            M.dom_lo[3 * m + d] = P.dom_lo[3 * b + d];
+            M.dom_hi[3 * m + d] = P.dom_hi[3 * b + d];
+            M.nc[3 * m + d] = P.nc[3 * b + d];
+            M.nf[3 * m + d] = P.nf[3 * b + d];
+            M.hc[3 * m + d] = P.hc[3 * b + d];
+            M.hf[3 * m + d] = P.hf[3 * b + d];
+        }
+        M.box_k[3 * m] = P.K.k.real();
+        M.box_k[3 * m + 1] = P.K.k.imag();
+        M.box_k[3 * m + 2] = P.K.kappa;
+        const int l = P.S.level_of(b);
+        if (l < int(part_of[g].size()))
+            M.box_up_part[m] = part_of[g][l];
+        M.table_off[m + 1] = M.table_off[m] + (P.table_off[b + 1] - P.table_off[b]);
+        for (int64_t c = P.cone_off[b]; c < P.cone_off[b + 1]; c++)
+        {
+            cone_map[g][c] = int32_t(M.cone_box.size());
+            M.cone_box.push_back(int32_t(m));
+            M.cone_cell.push_back(P.cone_cell[c]);
+            M.cone_mask.push_back(P.cone_mask[c]);
+            M.cone_fine.push_back(M.n_fine);
+            M.n_fine += __builtin_popcount(P.cone_mask[c]);
+        }
+        M.cone_off[m + 1] = int64_t(M.cone_box.size());
+    }
+    M.table.assign(M.table_off[nb], -1);
+    for (int64_t m = 0; m < nb; m++)
+    {
+        const Plan& P = *parts[origin[m].first];
+        const int64_t b = origin[m].second;
+        const int g = origin[m].first;
+        for (int64_t i = P.table_off[b]; i < P.table_off[b + 1]; i++)
+            if (P.table[i] >= 0)
+                M.table[M.table_off[m] + (i - P.table_off[b])] = cone_map[g][P.table[i]];
+    }
+    // active fine cone ids per box (introspection), in the new box order
+    M.fine_off.assign(nb + 1, 0);
+    for (int64_t m = 0; m < nb; m++)
+    {
+        const Plan& P = *parts[origin[m].first];
+        const int64_t b = origin[m].second;
+        if (int64_t(P.fine_off.size()) == P.S.num_boxes() + 1)
+            M.fine_ids.insert(M.fine_ids.end(), P.fine_ids.begin() + P.fine_off[b],
+                              P.fine_ids.begin() + P.fine_off[b + 1]);
+        M.fine_off[m + 1] = int64_t(M.fine_ids.size());
+    }
+    for (const Plan* p : parts)
+        M.n_fine_uncovered += p->n_fine_uncovered;
+    M.up.assign(ls, UpLevel()); // tables live in up_parts
+    return "";
+}
+
 } // namespace ifgf

@@ -143,6 +143,19 @@ struct Plan
     std::vector<UpLevel> up;
     int64_t up_budget = int64_t(64) << 20; // entries
 
+    // several geometries in one plan (merge_plans): the upward tables of each (geometry, level)
+    // with the parent boxes they serve, and the wavenumber of every source box (k_re, k_im,
+    // kappa).  Empty for a plan of one geometry (tables per level in `up`, P.K for all boxes).
+    struct UpPart
+    {
+        int level;
+        UpLevel U;
+    };
+    std::vector<UpPart> up_parts;
+    std::vector<int32_t> box_up_part; // per source box: its up_parts index or -1
+    std::vector<double> box_k;        // 3 per source box
+    std::vector<int64_t> part_src_off, part_tgt_off; // caller-order offsets of the geometries
+
     int64_t num_cones() const { return int64_t(cone_box.size()); }
 };
 
@@ -165,6 +178,12 @@ void build_cones(Plan& P, int nthreads);
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
                        const std::function<bool(double)>& cutoff, int nthreads);
+
+// One plan for several planned geometries of one kernel family and one set of orders: the
+// boxes of every level are those of part 0, then part 1, ...; points, cones, coarse cells and
+// pairs follow; sources and targets are the parts' concatenated in order.  Returns "" or an
+// error (families, orders or normals that differ).
+std::string merge_plans(const std::vector<const Plan*>& parts, Plan& M);
 
 // Chebyshev helpers shared by the host plan and the device tables
 std::vector<double> chebnodes1d(int n); // x_i = cos((2i+1) pi / 2n), chebinterp.hpp:27-39

@@ -200,6 +200,45 @@ class _IfgfOperator:
         return dict(zip(["near", "leaf", "upward", "transform", "far", "total"], t))
 
 
+class MergedIfgfOperator(_IfgfOperator):
+    """Several initialised operators of one class and one set of orders as one (ifgf_merge):
+    every phase of mult runs one launch per level for all geometries.  mult takes the parts'
+    densities concatenated (or a list of them, returning a list of results)."""
+
+    def __init__(self, ops, *, plan_only=False):
+        ops = list(ops)
+        if not ops or any(getattr(o, "_n", None) is None for o in ops):
+            raise _lib.IfgfError("merge needs initialised operators")
+        arr = (ctypes.c_void_p * len(ops))(*[o._h.value for o in ops])
+        h = ctypes.c_void_p()
+        rc = lib().ifgf_merge(arr, len(ops), _lib.INIT_PLAN_ONLY if plan_only else 0,
+                              ctypes.byref(h))
+        check(rc, ops[0]._h, "merge")
+        self._h = h
+        self._real = ops[0]._real
+        self._kernel = ops[0]._kernel
+        n, m = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().ifgf_sizes(h, ctypes.byref(n), ctypes.byref(m)), h, "sizes")
+        self._n, self._m = n.value, m.value
+        self._keep = None
+        self.parts = [(o._n, o._m) for o in ops]
+
+    def mult(self, weights):
+        if isinstance(weights, (list, tuple)):
+            u = super().mult(np.concatenate([np.asarray(w) for w in weights]))
+            out, at = [], 0
+            for _, m in self.parts:
+                out.append(u[at:at + m])
+                at += m
+            return out
+        return super().mult(weights)
+
+
+def merge(ops, *, plan_only=False):
+    """One operator for several initialised operators of one class and one set of orders."""
+    return MergedIfgfOperator(ops, plan_only=plan_only)
+
+
 class HelmholtzIfgfOperator(_IfgfOperator):
     """helmholtz_ifgf.hpp: G = e^{i kappa d} / (4 pi d), real kappa."""
     _kernel = _lib.HELMHOLTZ

@@ -80,6 +80,19 @@ int ifgf_mult(ifgf_operator* op, const double* w, int64_t nrhs, double* out, int
 int ifgf_mult_adjoint(ifgf_operator* op, const double* g, int64_t nrhs, double* out, int flags,
                       void* stream);
 
+/* Several operators of one kernel family and one set of interpolation orders, each after
+ * ifgf_init (IFGF_INIT_PLAN_ONLY is enough), as one operator on the device: every phase runs
+ * one launch per level for all of them.  mult takes the parts' densities concatenated in order
+ * (sum of N) and returns their results concatenated (sum of M); every part keeps its own
+ * wavenumber and points.  Precision and kernel family are those of ops[0].  No counterpart in
+ * the reference (one geometry per IfgfOperator); for batches of many geometries.  The parts
+ * stay the caller's; errors are reported on ops[0].  flags: IFGF_INIT_PLAN_ONLY merges the
+ * host plans only (introspection). */
+int ifgf_merge(ifgf_operator* const* ops, int n_ops, int flags, ifgf_operator** out);
+
+/* Numbers of sources and targets of a planned operator (a merged one: the sums). */
+int ifgf_sizes(const ifgf_operator* op, int64_t* n_src, int64_t* n_tgt);
+
 void ifgf_destroy(ifgf_operator* op);
 const char* ifgf_last_error(const ifgf_operator* op);
 const char* ifgf_version(void);

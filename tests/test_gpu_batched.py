@@ -135,3 +135,75 @@ def test_configuration3_spheroid_32_columns(gpu):
         err = rel(U[idx, c], ref)
         print(f"configuration 3: column {c} rel. error vs direct sum {err:.2e}", flush=True)
         assert err < 3e-3
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_merged_geometries_match_separate_operators(gpu, prec):
+    """Three geometries with their own wavenumbers merged into one operator (one launch per
+    level): each part's result against its own operator, the adjoint, and a direct sum."""
+    geos = [(sphere(6000, 111), 8.0), (spheroid(4000, 112) * 1.3, 15.0), (sphere(1500, 113), 3.0)]
+    ops, ws = [], []
+    for i, (x, k) in enumerate(geos):
+        op = ifgf_amd.HelmholtzIfgfOperator(k, 64, 6, 1, -1, precision=prec)
+        op.init(x.T, x.T)
+        ops.append(op)
+        ws.append(cdens(len(x), 114 + i))
+    m = ifgf_amd.merge(ops)
+    us = m.mult(ws)
+    for op, w, u in zip(ops, ws, us):
+        assert rel(u, op.mult(w)) < TOL_SAME[prec]
+    # the adjoint of the merged operator (targets = sources in every part)
+    gs = [cdens(len(x), 120 + i) for i, (x, _) in enumerate(geos)]
+    a = m.mult_adjoint(np.concatenate(gs))
+    at = 0
+    for op, g in zip(ops, gs):
+        assert rel(a[at:at + len(g)], op.mult_adjoint(g)) < TOL_SAME[prec]
+        at += len(g)
+    x, k = geos[2]
+    idx = np.arange(0, len(x), 7)
+    d = po.Oracle(po.HELMHOLTZ, k, 64, 6, 1, -1).direct_sum(x, x, ws[2], idx)
+    assert rel(us[2][idx], d) < 2e-3
+
+
+def test_configuration3_sixteen_spheres_one_launch_per_level(gpu):
+    """16 spheres, N log-uniform in [4096, 262144], kappa uniform in [8, 64], merged into one
+    operator (mixed form): every part against its own operator; timings of one merged mult
+    against the 16 separate mults."""
+    import time
+
+    import torch
+    rng = np.random.default_rng(1013)
+    Ns = np.floor(4096 * 64.0 ** rng.random(16)).astype(int)
+    ks = rng.uniform(8, 64, 16)
+    ops, ws = [], []
+    for i, (n, k) in enumerate(zip(Ns, ks)):
+        x = sphere(int(n), 1100 + i)
+        op = ifgf_amd.HelmholtzIfgfOperator(float(k), 64, 6, 1, -1)
+        op.init(x.T, x.T)
+        ops.append(op)
+        ws.append(cdens(int(n), 2100 + i))
+    m = ifgf_amd.merge(ops)
+    print(f"configuration 3: 16 spheres, N = {Ns.tolist()}, total {int(Ns.sum())}", flush=True)
+    wd = torch.as_tensor(np.concatenate(ws), device="cuda:0")
+    wds = [torch.as_tensor(w, device="cuda:0") for w in ws]
+    m.mult(wd)
+    for op, w in zip(ops, wds):
+        op.mult(w)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(3):
+        u = m.mult(wd)
+    torch.cuda.synchronize()
+    tm = (time.time() - t0) / 3
+    t0 = time.time()
+    for _ in range(3):
+        us = [op.mult(w) for op, w in zip(ops, wds)]
+    torch.cuda.synchronize()
+    ts = (time.time() - t0) / 3
+    print(f"configuration 3: merged mult {tm * 1e3:.1f} ms, 16 separate mults {ts * 1e3:.1f} ms",
+          flush=True)
+    u = u.cpu().numpy()
+    at = 0
+    for n, ui in zip(Ns, us):
+        assert rel(u[at:at + n], ui.cpu().numpy()) < TOL_SAME["mixed"]
+        at += n

@@ -160,3 +160,67 @@ def test_plan_errors():
     with pytest.raises(ValueError):
         ifgf_amd.DoubleLayerHelmholtzIfgfOperator(1j, 64, 8, 1, -1).init(
             sphere(100, 2).T, sphere(100, 2).T, plan_only=True)
+
+
+def test_merged_plan_is_the_parts_side_by_side():
+    """ifgf_merge (host plan): boxes level-major with the parts in order, pairs, cones and
+    point ranges of every part carried over with the id offsets recomputed here."""
+    geos = [(sphere(3000, 71), 8.0), (spheroid(2500, 72) * 1.3, 15.0), (sphere(900, 73), 3.0)]
+    ops = []
+    for x, k in geos:
+        op = ifgf_amd.HelmholtzIfgfOperator(k, 32, 6, 1, -1)
+        op.init(x.T, x.T, plan_only=True)
+        ops.append(op)
+    m = ifgf_amd.merge(ops, plan_only=True)
+    assert (m._n, m._m) == (sum(len(x) for x, _ in geos),) * 2
+    cm = m.plan_counts()
+    cs = [o.plan_counts() for o in ops]
+    for key in ("src_boxes", "far_pairs", "near_pairs", "coarse_cones", "fine_cones"):
+        assert cm[key] == sum(c[key] for c in cs), key
+    assert cm["src_levels"] == max(c["src_levels"] for c in cs)
+    # id maps: level-major, parts in order within a level
+    trees = [o.plan_tree(0) for o in ops]
+    tm = m.plan_tree(0)
+    L = cm["src_levels"]
+    maps, nxt, poff = [np.full(c["src_boxes"], -1) for c in cs], 0, [0]
+    for x, _ in geos:
+        poff.append(poff[-1] + len(x))
+    for lv in range(L):
+        assert tm["level_off"][lv] == nxt
+        for g, t in enumerate(trees):
+            if lv < len(t["level_off"]) - 1:
+                a, b = t["level_off"][lv], t["level_off"][lv + 1]
+                maps[g][a:b] = np.arange(nxt, nxt + (b - a))
+                nxt += b - a
+    for g, t in enumerate(trees):
+        mp = maps[g]
+        assert np.array_equal(tm["begin"][mp], t["begin"] + poff[g])
+        assert np.array_equal(tm["end"][mp], t["end"] + poff[g])
+        assert np.array_equal(tm["leaf"][mp], t["leaf"])
+        assert np.array_equal(tm["bbox"][mp], t["bbox"])
+        par = np.where(t["parent"] >= 0, mp[np.maximum(t["parent"], 0)], -1)
+        assert np.array_equal(tm["parent"][mp], par)
+        assert np.array_equal(tm["perm"][poff[g]:poff[g + 1]], t["perm"] + poff[g])
+    for far in (True, False):
+        s_m, t_m = m.plan_pairs(far)
+        got = set(zip(s_m.tolist(), t_m.tolist()))
+        want = set()
+        for g, o in enumerate(ops):
+            s_, t_ = o.plan_pairs(far)
+            want |= set(zip(maps[g][s_].tolist(), maps[g][t_].tolist()))
+        assert got == want
+    cm_ = m.plan_cones()
+    for g, o in enumerate(ops):
+        c = o.plan_cones()
+        for b in range(len(c["cone_off"]) - 1):
+            nb = maps[g][b]
+            cells = c["cell"][c["cone_off"][b]:c["cone_off"][b + 1]]
+            masks = c["mask"][c["cone_off"][b]:c["cone_off"][b + 1]]
+            assert np.array_equal(cm_["cell"][cm_["cone_off"][nb]:cm_["cone_off"][nb + 1]], cells)
+            assert np.array_equal(cm_["mask"][cm_["cone_off"][nb]:cm_["cone_off"][nb + 1]], masks)
+            assert np.array_equal(cm_["nc"][nb], c["nc"][b])
+    # different interpolation orders do not merge
+    other = ifgf_amd.HelmholtzIfgfOperator(8.0, 32, 8, 1, -1)
+    other.init(geos[0][0].T, geos[0][0].T, plan_only=True)
+    with pytest.raises(ifgf_amd.IfgfError):
+        ifgf_amd.merge([ops[0], other], plan_only=True)
