@@ -161,6 +161,13 @@ def main():
     ms, u = time_mults(op, w_dev, args.steps, args.warmup)
     print(f"# mixed: {ms:.3f} ms/matvec", file=sys.stderr, flush=True)
 
+    # the same mults replayed as the operator's captured HIP graph (kernels between the gather
+    # and the scatter)
+    op.use_graph = True
+    ms_graph, _ = time_mults(op, w_dev, args.steps, args.warmup)
+    op.use_graph = False
+    print(f"# mixed, graph replay: {ms_graph:.3f} ms/matvec", file=sys.stderr, flush=True)
+
     # per-phase device times: the timed steps repeated with the library's phase events (HIP
     # events on the mult's stream around every launch group, read once per mult)
     os.environ["IFGF_PHASE_TIMING"] = "1"
@@ -254,6 +261,7 @@ def main():
             "densities": "complex N(0,1), seed 2002; points seed 1002",
             "parallelism": "single GPU",
         },
+        "graph_ms_per_step": ms_graph,
         "rel_err_vs_direct": err,
         "rel_err_samples": int(len(idx)),
         "init_s": t_init,

@@ -20,6 +20,7 @@ HELMHOLTZ, MODHELMHOLTZ, LAPLACE, DOUBLELAYER, COMBINED, GRAD = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_PLAN_ONLY = 1
 DEVICE_POINTERS = 1
+MULT_GRAPH = 2
 
 CUTOFF_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_double, ctypes.c_void_p)
 

@@ -148,7 +148,7 @@ extern "C"
         if (!w || !out || nrhs < 1)
             return fail(op, IFGF_EINVAL, "mult needs weights, output and nrhs >= 1");
         std::string e = device_mult(op->dev, w, out, nrhs, (flags & IFGF_DEVICE_POINTERS) != 0,
-                                    stream);
+                                    stream, 0, (flags & IFGF_MULT_GRAPH) != 0);
         if (!e.empty())
             return fail(op, IFGF_EINTERNAL, e);
         return IFGF_OK;

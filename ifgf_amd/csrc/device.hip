@@ -845,9 +845,9 @@ struct PhaseTimer
     }
 };
 
-template <int KIND, typename R>
-void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed,
-              int io)
+// everything of a mult between the gather of the densities and the scatter of the results:
+// it reads and writes only the plan's buffers, so one captured graph serves every call
+template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt)
 {
     constexpr bool PH = KIND != KL;
     const KParams kp = kparams(D);
@@ -856,13 +856,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
-    PhaseTimer pt{timed, st, {}};
-    const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
-    const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
-    const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
     pt.begin(0);
-    k_gather<R><<<dim3(unsigned((D.nsrc + 255) / 256)), dim3(256), 0, st>>>(w_dev, D.src_perm, ws,
-                                                                           D.nsrc, in_im);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
     if (D.n_near_items)
         k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
@@ -900,6 +894,46 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
             launch_up_far<PH, R>(D, st, kp, tabs, -1, int(l), a, b, F, V, res);
     }
     pt.end();
+}
+
+// one mult: gather, the core (eagerly, or as the plan's captured graph), scatter
+template <int KIND, typename R>
+void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed,
+              int io, bool graph)
+{
+    Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
+    Cx<R>* res = static_cast<Cx<R>*>(D.result);
+    PhaseTimer pt{timed, st, {}};
+    const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
+    const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
+    const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
+    pt.begin(0);
+    k_gather<R><<<dim3(unsigned((D.nsrc + 255) / 256)), dim3(256), 0, st>>>(w_dev, D.src_perm, ws,
+                                                                           D.nsrc, in_im);
+    pt.end();
+    if (graph && !timed)
+    {
+        if (!D.graph_exec)
+        {
+            // capture on a private stream (the caller's may be the null stream, which cannot
+            // be captured); kernels only, no memory set nodes
+            hipStream_t cs;
+            HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            D.cap_stream = cs;
+            PhaseTimer none{false, cs, {}};
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            run_core<KIND, R>(D, cs, none);
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamEndCapture(cs, &g));
+            hipGraphExec_t ex = nullptr;
+            HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+            D.graph = g;
+            D.graph_exec = ex;
+        }
+        HIP_OK(hipGraphLaunch(static_cast<hipGraphExec_t>(D.graph_exec), st));
+    }
+    else
+        run_core<KIND, R>(D, st, pt);
     pt.begin(0);
     k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(
         res, D.tgt_perm, out_dev, D.ntgt, out_re, out_im);
@@ -909,22 +943,23 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
 }
 
 template <typename R>
-void dispatch(DevicePlan& D, const double* w, double* o, hipStream_t st, bool timed, int io)
+void dispatch(DevicePlan& D, const double* w, double* o, hipStream_t st, bool timed, int io,
+              bool graph)
 {
     switch (D.kind)
     {
     case KIND_HELMHOLTZ:
-        return run_mult<KH, R>(D, w, o, st, timed, io);
+        return run_mult<KH, R>(D, w, o, st, timed, io, graph);
     case KIND_MODHELMHOLTZ:
-        return run_mult<KM, R>(D, w, o, st, timed, io);
+        return run_mult<KM, R>(D, w, o, st, timed, io, graph);
     case KIND_LAPLACE:
-        return run_mult<KL, R>(D, w, o, st, timed, io);
+        return run_mult<KL, R>(D, w, o, st, timed, io, graph);
     case KIND_DOUBLELAYER:
-        return run_mult<KD, R>(D, w, o, st, timed, io);
+        return run_mult<KD, R>(D, w, o, st, timed, io, graph);
     case KIND_COMBINED:
-        return run_mult<KC, R>(D, w, o, st, timed, io);
+        return run_mult<KC, R>(D, w, o, st, timed, io, graph);
     case KIND_GRAD:
-        return run_mult<KG, R>(D, w, o, st, timed, io);
+        return run_mult<KG, R>(D, w, o, st, timed, io, graph);
     }
     throw std::runtime_error("unknown kernel kind");
 }
@@ -950,6 +985,12 @@ void device_free(DevicePlan& D)
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
+    if (D.graph_exec)
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.graph_exec));
+    if (D.graph)
+        (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
+    if (D.cap_stream)
+        (void)hipStreamDestroy(static_cast<hipStream_t>(D.cap_stream));
     D = DevicePlan();
 }
 
@@ -1336,7 +1377,7 @@ static bool ensure_batch(DevicePlan& D, int rb, bool host_io)
 }
 
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
-                        void* stream, int io)
+                        void* stream, int io, bool graph)
 {
     try
     {
@@ -1393,9 +1434,9 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 odev = obuf;
             }
             if (D.prec == PREC_FP64)
-                dispatch<double>(D, wdev, odev, st, timed, io);
+                dispatch<double>(D, wdev, odev, st, timed, io, graph);
             else
-                dispatch<float>(D, wdev, odev, st, timed, io);
+                dispatch<float>(D, wdev, odev, st, timed, io, graph);
             if (!device_ptrs)
                 HIP_OK(hipMemcpyAsync(orr, obuf, 16 * size_t(D.ntgt), hipMemcpyDeviceToHost, st));
         }

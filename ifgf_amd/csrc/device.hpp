@@ -127,6 +127,11 @@ struct DevicePlan
 
     // work buffers of the batched mult (b_rb columns, column-minor), allocated on first use;
     // b_in / b_out stage host densities and results
+    // the single-column mult's core captured as a graph (on first use with graphs enabled)
+    void* graph = nullptr;      // hipGraph_t
+    void* graph_exec = nullptr; // hipGraphExec_t
+    void* cap_stream = nullptr; // hipStream_t of the capture
+
     int b_rb = 0;
     void *b_ws = nullptr, *b_res = nullptr, *b_V = nullptr, *b_F = nullptr;
     double *b_in = nullptr, *b_out = nullptr;
@@ -150,7 +155,7 @@ enum
     IO_NEGATE_OUT = 4
 };
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
-                        void* stream, int io = 0);
+                        void* stream, int io = 0, bool graph = false);
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
 void device_last_timings(const DevicePlan& D, double* out6);
 bool device_available();

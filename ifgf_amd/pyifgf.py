@@ -58,6 +58,8 @@ class _IfgfOperator:
         self._n = None
         self._m = None
         self._keep = None
+        # single-column mults of device tensors replay the mult's kernels as a HIP graph
+        self.use_graph = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -121,8 +123,11 @@ class _IfgfOperator:
         wc = w2.t().to(torch.complex128).resolve_conj().resolve_neg().contiguous()
         out = torch.empty((nrhs, n_out), dtype=torch.complex128, device=w.device)
         stream = torch.cuda.current_stream(w.device).cuda_stream
-        check(getattr(lib(), fname)(self._h, wc.data_ptr(), nrhs, out.data_ptr(),
-                                    _lib.DEVICE_POINTERS, stream), self._h, fname[5:])
+        flags = _lib.DEVICE_POINTERS
+        if getattr(self, "use_graph", False) and fname == "ifgf_mult":
+            flags |= _lib.MULT_GRAPH
+        check(getattr(lib(), fname)(self._h, wc.data_ptr(), nrhs, out.data_ptr(), flags, stream),
+              self._h, fname[5:])
         res = out.t()
         if self._real:
             res = res.real

@@ -41,6 +41,8 @@ extern "C" {
 
 /* ifgf_mult flags */
 #define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */
+#define IFGF_MULT_GRAPH 2      /* one column: replay the mult's kernels as a captured HIP graph
+                                  (captured on the first such call; reused while the operator lives) */
 
 typedef struct ifgf_operator ifgf_operator;
 

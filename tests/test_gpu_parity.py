@@ -165,3 +165,25 @@ def test_configuration1_error_vs_direct_sum(gpu):
         err = rel(u[idx], d)
         print(f"configuration 1 {prec}: rel. error vs direct sum {err:.3e}", flush=True)
         assert err < 5e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_graph_replays_match_eager(gpu, prec):
+    """The mult's kernels replayed as a captured HIP graph: every replay, with eager work of
+    another operator and of torch and synchronisations in between, equals the eager mult."""
+    import torch
+    x = sphere(8000, 130)
+    op = ifgf_amd.HelmholtzIfgfOperator(10.0, 64, 6, 1, -1, precision=prec)
+    op.init(x.T, x.T)
+    other = ifgf_amd.HelmholtzIfgfOperator(5.0, 32, 6, 1, -1, precision=prec)
+    other.init(x.T, x.T)
+    ws = [torch.as_tensor(cdens(8000, 131 + i), device="cuda:0") for i in range(3)]
+    eager = [op.mult(w).cpu() for w in ws]
+    op.use_graph = True
+    for rep in range(3):
+        for i, w in enumerate(ws):
+            u = op.mult(w)
+            other.mult(w)
+            (w * 2).sum()
+            torch.cuda.synchronize()
+            assert torch.equal(u.cpu(), eager[i]), (rep, i)
