@@ -159,4 +159,8 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
 void device_last_timings(const DevicePlan& D, double* out6);
 bool device_available();
+
+// The source or target octree of init on the device (tree.hip): the same Tree as build_tree
+// (identical permutation, boxes and point ranges).  Returns "" or an error.
+std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T);
 } // namespace ifgf

@@ -204,17 +204,17 @@ int Tree::level_of(int64_t g) const
     return l;
 }
 
-void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nthreads)
+void tree_root(Tree& T, int64_t n, const double mn_in[3], const double mx_in[3], double ctr[3],
+               double diam[3])
 {
     T = Tree();
     T.n = n;
-    double mn[3] = {pts[0], pts[1], pts[2]}, mx[3] = {pts[0], pts[1], pts[2]};
-    for (int64_t i = 0; i < n; i++)
-        for (int d = 0; d < 3; d++)
-        {
-            mn[d] = std::min(mn[d], pts[3 * i + d]);
-            mx[d] = std::max(mx[d], pts[3 * i + d]);
-        }
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; d++)
+    {
+        mn[d] = mn_in[d];
+        mx[d] = mx_in[d];
+    }
     const double a = 1e-8 * nrm(mn), b = 1e-8 * nrm(mx);
     for (int d = 0; d < 3; d++)
     {
@@ -223,12 +223,104 @@ void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nth
         T.lo[d] = mn[d];
         T.hi[d] = mx[d];
     }
-    double ctr[3], diam[3];
     for (int d = 0; d < 3; d++)
     {
         ctr[d] = (mx[d] + mn[d]) / 2.0;
         diam[d] = mx[d] - mn[d];
     }
+}
+
+void child_box(const double* blo, const double* bhi, int j, double lo[3], double hi[3])
+{
+    for (int d = 0; d < 3; d++)
+    {
+        double size = 0.5 * (bhi[d] - blo[d]);
+        double tup = double((j >> d) & 1);
+        lo[d] = blo[d] + size * tup;
+        hi[d] = lo[d] + size;
+    }
+}
+
+void grow_tree(Tree& T, int64_t max_leaf, const LevelSplit& split_level)
+{
+    const int64_t n = T.n;
+    // levels: children of every box with more than max_leaf points, greedy closed-box scan
+    auto add_box = [&](int64_t b, int64_t e, int32_t parent, const double lo[3], const double hi[3])
+    {
+        T.begin.push_back(b);
+        T.end.push_back(e);
+        T.parent.push_back(parent);
+        for (int j = 0; j < 8; j++)
+            T.child.push_back(-1);
+        T.leaf.push_back(1);
+        double m = 0;
+        for (int d = 0; d < 3; d++)
+        {
+            T.blo.push_back(lo[d]);
+            T.bhi.push_back(hi[d]);
+            T.ctr.push_back((hi[d] + lo[d]) / 2.0);
+            m = std::max(m, std::abs(hi[d] - lo[d]));
+        }
+        T.side.push_back(m);
+    };
+    T.level_off.assign(1, 0);
+    T.begin.clear();
+    T.end.clear();
+    T.parent.clear();
+    T.child.clear();
+    T.leaf.clear();
+    T.blo.clear();
+    T.bhi.clear();
+    T.ctr.clear();
+    T.side.clear();
+    add_box(0, n, -1, T.lo, T.hi);
+    int64_t lb = 0, le = 1;
+    T.levels = 1;
+    std::vector<ChildRange> cr;
+    std::vector<uint8_t> split;
+    while (true)
+    {
+        cr.assign(le - lb, ChildRange());
+        split.assign(le - lb, 0);
+        split_level(T, lb, le, max_leaf, cr, split);
+        const int64_t nb0 = T.num_boxes();
+        for (int64_t q = 0; q < le - lb; q++)
+        {
+            if (!split[q])
+                continue;
+            const int64_t g = lb + q;
+            for (int j = 0; j < 8; j++)
+            {
+                if (cr[q].b[j] == cr[q].e[j])
+                    continue;
+                double cl[3], ch[3];
+                child_box(&T.blo[3 * g], &T.bhi[3 * g], j, cl, ch);
+                T.child[8 * g + j] = int32_t(T.num_boxes());
+                T.leaf[g] = 0;
+                add_box(cr[q].b[j], cr[q].e[j], int32_t(g), cl, ch);
+            }
+        }
+        if (T.num_boxes() == nb0)
+            break;
+        T.level_off.push_back(nb0);
+        T.levels++;
+        lb = nb0;
+        le = T.num_boxes();
+    }
+    T.level_off.push_back(T.num_boxes());
+}
+
+void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nthreads)
+{
+    double mn[3] = {pts[0], pts[1], pts[2]}, mx[3] = {pts[0], pts[1], pts[2]};
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++)
+        {
+            mn[d] = std::min(mn[d], pts[3 * i + d]);
+            mx[d] = std::max(mx[d], pts[3 * i + d]);
+        }
+    double ctr[3], diam[3];
+    tree_root(T, n, mn, mx, ctr, diam);
 
     // sort by 126-bit keys, then order equal-key runs with the exact comparison
     struct KI
@@ -274,99 +366,38 @@ void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nth
     keys.clear();
     keys.shrink_to_fit();
 
-    // levels: children of every box with more than max_leaf points, greedy closed-box scan
-    auto add_box = [&](int64_t b, int64_t e, int32_t parent, const double lo[3], const double hi[3])
-    {
-        T.begin.push_back(b);
-        T.end.push_back(e);
-        T.parent.push_back(parent);
-        for (int j = 0; j < 8; j++)
-            T.child.push_back(-1);
-        T.leaf.push_back(1);
-        double m = 0;
-        for (int d = 0; d < 3; d++)
-        {
-            T.blo.push_back(lo[d]);
-            T.bhi.push_back(hi[d]);
-            T.ctr.push_back((hi[d] + lo[d]) / 2.0);
-            m = std::max(m, std::abs(hi[d] - lo[d]));
-        }
-        T.side.push_back(m);
-    };
-    T.level_off.push_back(0);
-    add_box(0, n, -1, mn, mx);
-    int64_t lb = 0, le = 1;
-    T.levels = 1;
-    struct ChildRange
-    {
-        int64_t b[8], e[8];
-        double lo[8][3], hi[8][3];
-    };
-    while (true)
-    {
-        std::vector<ChildRange> cr(le - lb);
-        std::vector<uint8_t> split(le - lb, 0);
-        pfor(le - lb, nthreads,
-             [&](int64_t q, int)
-             {
-                 const int64_t g = lb + q;
-                 const int64_t b = T.begin[g], e = T.end[g];
-                 if (e - b <= max_leaf)
-                     return;
-                 split[q] = 1;
-                 int64_t pi = b;
-                 for (int j = 0; j < 8; j++)
-                 {
-                     double cl[3], ch[3];
-                     for (int d = 0; d < 3; d++)
-                     {
-                         double size = 0.5 * (T.bhi[3 * g + d] - T.blo[3 * g + d]);
-                         double tup = double((j >> d) & 1);
-                         cl[d] = T.blo[3 * g + d] + size * tup;
-                         ch[d] = cl[d] + size;
-                     }
-                     int64_t ep = pi;
-                     while (ep < e)
-                     {
-                         const double* p = &T.pts[3 * ep];
-                         if (!(cl[0] <= p[0] && cl[1] <= p[1] && cl[2] <= p[2] && p[0] <= ch[0] &&
-                               p[1] <= ch[1] && p[2] <= ch[2]))
-                             break;
-                         ++ep;
-                     }
-                     cr[q].b[j] = pi;
-                     cr[q].e[j] = ep;
-                     for (int d = 0; d < 3; d++)
-                     {
-                         cr[q].lo[j][d] = cl[d];
-                         cr[q].hi[j][d] = ch[d];
-                     }
-                     pi = ep;
-                 }
-             });
-        const int64_t nb0 = T.num_boxes();
-        for (int64_t q = 0; q < le - lb; q++)
-        {
-            if (!split[q])
-                continue;
-            const int64_t g = lb + q;
-            for (int j = 0; j < 8; j++)
-            {
-                if (cr[q].b[j] == cr[q].e[j])
-                    continue;
-                T.child[8 * g + j] = int32_t(T.num_boxes());
-                T.leaf[g] = 0;
-                add_box(cr[q].b[j], cr[q].e[j], int32_t(g), cr[q].lo[j], cr[q].hi[j]);
-            }
-        }
-        if (T.num_boxes() == nb0)
-            break;
-        T.level_off.push_back(nb0);
-        T.levels++;
-        lb = nb0;
-        le = T.num_boxes();
-    }
-    T.level_off.push_back(T.num_boxes());
+    grow_tree(T, max_leaf,
+              [nthreads](const Tree& T, int64_t lb, int64_t le, int64_t max_leaf,
+                         std::vector<ChildRange>& cr, std::vector<uint8_t>& split)
+              {
+                  pfor(le - lb, nthreads,
+                       [&](int64_t q, int)
+                       {
+                           const int64_t g = lb + q;
+                           const int64_t b = T.begin[g], e = T.end[g];
+                           if (e - b <= max_leaf)
+                               return;
+                           split[q] = 1;
+                           int64_t pi = b;
+                           for (int j = 0; j < 8; j++)
+                           {
+                               double cl[3], ch[3];
+                               child_box(&T.blo[3 * g], &T.bhi[3 * g], j, cl, ch);
+                               int64_t ep = pi;
+                               while (ep < e)
+                               {
+                                   const double* p = &T.pts[3 * ep];
+                                   if (!(cl[0] <= p[0] && cl[1] <= p[1] && cl[2] <= p[2] &&
+                                         p[0] <= ch[0] && p[1] <= ch[1] && p[2] <= ch[2]))
+                                       break;
+                                   ++ep;
+                               }
+                               cr[q].b[j] = pi;
+                               cr[q].e[j] = ep;
+                               pi = ep;
+                           }
+                       });
+              });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1082,7 +1113,8 @@ void build_cones(Plan& P, int nthreads)
 
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
-                       const std::function<bool(double)>& cutoff, int nthreads)
+                       const std::function<bool(double)>& cutoff, int nthreads,
+                       const TreeBuilder& tree_builder)
 {
     if (N <= 0 || M <= 0)
         return "empty point set";
@@ -1094,8 +1126,19 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
     P.base_els[1] = 2 * int64_t(n_elem);
     P.base_els[2] = 4 * int64_t(n_elem);
     int64_t max_leaf = leaf < 0 ? std::numeric_limits<int64_t>::max() : leaf;
-    build_tree(srcs, N, max_leaf, P.S, nthreads);
-    build_tree(tgts, M, max_leaf, P.T, nthreads);
+    if (tree_builder)
+    {
+        std::string e = tree_builder(srcs, N, max_leaf, P.S);
+        if (e.empty())
+            e = tree_builder(tgts, M, max_leaf, P.T);
+        if (!e.empty())
+            return e;
+    }
+    else
+    {
+        build_tree(srcs, N, max_leaf, P.S, nthreads);
+        build_tree(tgts, M, max_leaf, P.T, nthreads);
+    }
     for (int d = 0; d < 3; d++)
         if (!(P.S.hi[d] > P.S.lo[d]) || !(P.T.hi[d] > P.T.lo[d]))
             return "degenerate bounding box (all points share a coordinate plane at the origin)";

@@ -169,7 +169,25 @@ void morton_key(const double p[3], const double ctr[3], const double diam[3], ui
 bool morton_less_exact(const double* px, const double* qx, const double ctr[3],
                        const double diam[3]);
 
+// The tree in three steps (octree.hpp:202-238, 1106-1199): tree_root pads the bounding box
+// [mn, mx] (boundingbox.hpp:13-26) and gives the Morton normalisation (ctr, diam); the points
+// are then sorted (perm, pts); grow_tree adds levels until no box has more than max_leaf
+// points, asking split_level for the child point ranges of each level (the greedy scan of
+// the closed child boxes, lower octant first) and forming the child boxes with child_box.
+struct ChildRange
+{
+    int64_t b[8], e[8];
+};
+using LevelSplit = std::function<void(const Tree& T, int64_t lb, int64_t le, int64_t max_leaf,
+                                      std::vector<ChildRange>& cr, std::vector<uint8_t>& split)>;
+void tree_root(Tree& T, int64_t n, const double mn[3], const double mx[3], double ctr[3],
+               double diam[3]);
+void child_box(const double* blo, const double* bhi, int j, double lo[3], double hi[3]);
+void grow_tree(Tree& T, int64_t max_leaf, const LevelSplit& split_level);
 void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nthreads);
+// builds the trees when set (the device builder); build_tree otherwise
+using TreeBuilder = std::function<std::string(const double* pts, int64_t n, int64_t max_leaf,
+                                              Tree& T)>;
 void build_interactions(Plan& P, const std::function<bool(double)>& cutoff);
 int estimate_orders(Plan& P, double tol, int out[3]); // ifgfoperator.hpp:108-244
 void build_cones(Plan& P, int nthreads);
@@ -177,7 +195,8 @@ void build_cones(Plan& P, int nthreads);
 // Build everything init() needs (ifgfoperator.hpp:65-106).  Returns "" or an error.
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
-                       const std::function<bool(double)>& cutoff, int nthreads);
+                       const std::function<bool(double)>& cutoff, int nthreads,
+                       const TreeBuilder& tree_builder = nullptr);
 
 // One plan for several planned geometries of one kernel family and one set of orders: the
 // boxes of every level are those of part 0, then part 1, ...; points, cones, coarse cells and

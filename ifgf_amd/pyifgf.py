@@ -68,7 +68,8 @@ class _IfgfOperator:
             self._h = None
 
     # IfgfOperator::init (ifgfoperator.hpp:65-106)
-    def init(self, srcs, targets, cutOff=None, normals=None, *, plan_only=False):
+    def init(self, srcs, targets, cutOff=None, normals=None, *, plan_only=False,
+             host_tree=False):
         s = _points(srcs, "srcs")
         t = _points(targets, "targets")
         n = None if normals is None else _points(normals, "normals")
@@ -77,7 +78,8 @@ class _IfgfOperator:
         self._keep = cb
         rc = lib().ifgf_init(self._h, s.ctypes.data, s.shape[0], t.ctypes.data, t.shape[0],
                              None if n is None else n.ctypes.data, cb, None,
-                             _lib.INIT_PLAN_ONLY if plan_only else 0)
+                             (_lib.INIT_PLAN_ONLY if plan_only else 0)
+                             | (_lib.INIT_HOST_TREE if host_tree else 0))
         check(rc, self._h, "init")
         self._n, self._m = s.shape[0], t.shape[0]
 
