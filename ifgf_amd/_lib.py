@@ -46,6 +46,9 @@ SIGNATURES = {
     "ifgf_plan_cones": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ifgf_plan_work": (_I, [_P, _P]),
     "ifgf_last_timings": (_I, [_P, _P]),
+    "ifgf_autotune": (_I, [_P, _I64, _I, _P]),
+    "ifgf_get_launch_shape": (_I, [_P, _P]),
+    "ifgf_set_launch_shape": (_I, [_P, _P]),
 }
 
 _lib = None

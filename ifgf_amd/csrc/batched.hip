@@ -702,9 +702,24 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
         const int64_t i0 = D.up_item_off[l], i1 = D.up_item_off[l + 1];
         if (i1 > i0)
-            k_up_cell_b<R, O0, O1, O2, UP_K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(
-                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[l], D.cones, D.geo, D.child,
-                D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp);
+        {
+#define IFGF_UP_CELL_B(K)                                                                       \
+    k_up_cell_b<R, O0, O1, O2, K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(           \
+        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[l], D.cones, D.geo, D.child, D.ctab, \
+        static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp)
+            switch (D.shape.up_k)
+            {
+            case 4:
+                IFGF_UP_CELL_B(4);
+                break;
+            case 16:
+                IFGF_UP_CELL_B(16);
+                break;
+            default:
+                IFGF_UP_CELL_B(8);
+            }
+#undef IFGF_UP_CELL_B
+        }
         if (u1 > u0)
             k_up_b<PH, R, O0, O1, O2, RB><<<dim3(unsigned(u1 - u0)), dim3(256), 0, st>>>(
                 D.up_cones + u0, u1 - u0, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
@@ -757,6 +772,8 @@ void run_order(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t 
         run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io);
 }
 } // namespace
+
+void set_xcd_log2_batched(int lg) { dev::set_xcd_log2(lg); }
 
 bool batched_supported(const DevicePlan& D)
 {

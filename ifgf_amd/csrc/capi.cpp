@@ -499,4 +499,48 @@ extern "C"
         device_last_timings(op->dev, ms6);
         return IFGF_OK;
     }
+
+    int ifgf_get_launch_shape(const ifgf_operator* op, int* shape4)
+    {
+        if (!op || !shape4)
+            return IFGF_EINVAL;
+        const LaunchShape& s = op->dev.shape;
+        shape4[0] = s.xcd_log2;
+        shape4[1] = s.cpb;
+        shape4[2] = s.up_k;
+        shape4[3] = s.rb;
+        return IFGF_OK;
+    }
+
+    int ifgf_set_launch_shape(ifgf_operator* op, const int* shape4)
+    {
+        if (!op || !shape4)
+            return IFGF_EINVAL;
+        if (!op->on_device)
+            return fail(op, IFGF_ESTATE, "launch shapes need a device init");
+        LaunchShape s;
+        s.xcd_log2 = shape4[0];
+        s.cpb = shape4[1];
+        s.up_k = shape4[2];
+        s.rb = shape4[3];
+        std::string e = device_set_shape(op->dev, s);
+        if (!e.empty())
+            return fail(op, IFGF_EINVAL, e);
+        return IFGF_OK;
+    }
+
+    int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape4)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!op->on_device)
+            return fail(op, IFGF_ESTATE, "autotune needs a device init");
+        if (nrhs < 1 || reps < 1)
+            return fail(op, IFGF_EINVAL, "autotune needs nrhs >= 1 and reps >= 1");
+        LaunchShape best;
+        std::string e = device_autotune(op->dev, nrhs, reps, best);
+        if (!e.empty())
+            return fail(op, IFGF_EINTERNAL, e);
+        return shape4 ? ifgf_get_launch_shape(op, shape4) : IFGF_OK;
+    }
 }

@@ -57,7 +57,6 @@ constexpr double PI_D = 3.14159265358979323846;
 constexpr double TWO_PI_D = 6.28318530717958647692;
 constexpr double INV_2PI_D = 0.15915494309189533577;
 constexpr double INV_4PI_D = 0.07957747154594766788;
-constexpr int UP_K = 8; // parent cones of one cell per upward workgroup
 
 enum
 {
@@ -701,19 +700,33 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
 // groups splitting the interaction list; reduction over groups by xor shuffles.
 // Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8) and every XCD has its
 // own L2.  Neighbouring work items read much of the same data (the same source boxes, the same
-// children, the same cells), so each XCD takes runs of XCD_RUN consecutive items, the runs
-// dealt out round-robin so that costs drifting along a work list stay balanced: workgroup b
-// handles item xcd_block(), a bijection of [0, gridDim.x) (the tail beyond the last whole
-// round of runs keeps its order).
-constexpr int64_t XCD_RUN = 16;
+// children, the same cells), so each XCD takes runs of 2^c_xcd_log2 consecutive items, the
+// runs dealt out round-robin so that costs drifting along a work list stay balanced: workgroup
+// b handles item xcd_block(), a bijection of [0, gridDim.x) for every run length (the tail
+// beyond the last whole round of runs keeps its order; run length 1 is the plain order).  The
+// run length is a launch shape the autotuner picks: each translation unit holds its own copy,
+// set from the host by set_xcd_log2 before a mult.
+static __constant__ int c_xcd_log2 = 4;
 __device__ __forceinline__ int64_t xcd_block()
 {
     const int64_t n = gridDim.x, b = blockIdx.x;
-    const int64_t full = n / (8 * XCD_RUN) * (8 * XCD_RUN);
+    const int lg = c_xcd_log2;
+    const int64_t full = (n >> (3 + lg)) << (3 + lg);
     if (b >= full)
         return b;
     const int64_t x = b & 7, i = b >> 3;
-    return ((i / XCD_RUN) * 8 + x) * XCD_RUN + i % XCD_RUN;
+    return ((((i >> lg) << 3) + x) << lg) + (i & ((int64_t(1) << lg) - 1));
+}
+
+// host side of one translation unit's run length (the constant is written only on a change)
+static int h_xcd_log2 = 4;
+static inline void set_xcd_log2(int lg)
+{
+    if (lg == h_xcd_log2)
+        return;
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_xcd_log2), &lg, sizeof(int), 0,
+                             hipMemcpyHostToDevice));
+    h_xcd_log2 = lg;
 }
 
 __device__ __forceinline__ int pow2_at_least(int n)

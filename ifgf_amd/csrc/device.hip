@@ -697,6 +697,7 @@ struct Timing
     float ms[6] = {0, 0, 0, 0, 0, 0};
 };
 Timing g_last;
+bool g_force_timing = false; // phase timing on (the autotuner reads single phases)
 
 
 // up_level >= 0: the upward pass into parent level up_level; otherwise the far field of
@@ -711,10 +712,24 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     {
         const int64_t i0 = D.up_item_off[up_level], i1 = D.up_item_off[up_level + 1];
         if (i1 > i0)
-            k_up_cell<R, O0, O1, O2, UP_K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
-                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,
-                D.child, D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F,
-                V, kp);
+        {
+#define IFGF_UP_CELL(K)                                                                         \
+    k_up_cell<R, O0, O1, O2, K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                \
+        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo, D.child,  \
+        D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp)
+            switch (D.shape.up_k)
+            {
+            case 4:
+                IFGF_UP_CELL(4);
+                break;
+            case 16:
+                IFGF_UP_CELL(16);
+                break;
+            default:
+                IFGF_UP_CELL(8);
+            }
+#undef IFGF_UP_CELL
+        }
         if (b > a)
             k_up<PH, R, O0, O1, O2><<<dim3(unsigned(b - a)), dim3(128), 0, st>>>(
                 D.up_cones + a, b - a, D.cones, D.geo, D.child, D.ctab, tabs, F, V, kp);
@@ -752,28 +767,83 @@ void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeT
     }
 }
 
+// transform cones per workgroup the compiled kernels offer per (order class, real size), the
+// default first (LDS: 6.5 KB per cone at order 6 in FP32, 16.5 KB at order 8)
+static std::vector<int> transform_cpb_choices(int cls, size_t rsize)
+{
+    if (cls == 1)
+        return rsize == 4 ? std::vector<int>{6, 2, 4, 8} : std::vector<int>{3, 1, 2, 4};
+    if (cls == 2)
+        return rsize == 4 ? std::vector<int>{2, 1, 3} : std::vector<int>{1};
+    return {0};
+}
+
 template <typename R>
 void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t c0, int64_t c1,
                       Cx<R>* V, Cx<R>* F)
 {
     const R* W = static_cast<const R*>(D.wmat);
     const int cls = order_class(D);
+#define IFGF_TRANSFORM(H0, H1, H2, L0, L1, L2, CPB)                                            \
+    k_transform_t<R, H0, H1, H2, L0, L1, L2, 3, 3, 3, CPB>                                      \
+        <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W)
+    const int cpb_sel =
+        D.shape.cpb > 0 ? D.shape.cpb : transform_cpb_choices(cls, sizeof(R))[0];
     if (cls == 1)
     {
-        constexpr int CPB = sizeof(R) == 4 ? 6 : 3;
-        k_transform_t<R, 3, 6, 6, 2, 3, 3, 3, 3, 3, CPB>
-            <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V,
-                                                                             F, W);
+        switch (cpb_sel)
+        {
+        case 1:
+            IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 1);
+            break;
+        case 2:
+            IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 2);
+            break;
+        case 3:
+            IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 3);
+            break;
+        case 4:
+            IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 4);
+            break;
+        case 8:
+            if constexpr (sizeof(R) == 4)
+            {
+                IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 8);
+                break;
+            }
+            [[fallthrough]];
+        default:
+            if constexpr (sizeof(R) == 4)
+                IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 6);
+            else
+                IFGF_TRANSFORM(3, 6, 6, 2, 3, 3, 3);
+        }
         return;
     }
     if (cls == 2)
     {
-        constexpr int CPB = sizeof(R) == 4 ? 2 : 1;
-        k_transform_t<R, 5, 8, 8, 2, 5, 5, 3, 3, 3, CPB>
-            <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V,
-                                                                             F, W);
+        switch (cpb_sel)
+        {
+        case 3:
+            if constexpr (sizeof(R) == 4)
+            {
+                IFGF_TRANSFORM(5, 8, 8, 2, 5, 5, 3);
+                break;
+            }
+            [[fallthrough]];
+        case 2:
+            if constexpr (sizeof(R) == 4)
+            {
+                IFGF_TRANSFORM(5, 8, 8, 2, 5, 5, 2);
+                break;
+            }
+            [[fallthrough]];
+        default:
+            IFGF_TRANSFORM(5, 8, 8, 2, 5, 5, 1);
+        }
         return;
     }
+#undef IFGF_TRANSFORM
     const int h0 = D.ho[0], h1 = D.ho[1], h2 = D.ho[2];
     const int m0 = D.f[0] * D.lo[0], m1 = D.f[1] * D.lo[1], m2 = D.f[2] * D.lo[2];
     // X holds the values and the axis-1 stage, Y the axis-0 stage; axis 2 goes to memory
@@ -978,7 +1048,8 @@ void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
-                    D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items, D.up_glist,
+                    D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items_k[0],
+                    D.up_items_k[1], D.up_items_k[2], D.up_glist,
                     D.leaf_cones, D.up_cones, D.far_off, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
@@ -1218,13 +1289,14 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                 D.up_entries = dupload(ef, tot);
         }
         // work lists.  Per level: leaf cones (leaf sampling), parent cones of boxes with a table
-        // grouped by (part, cell) into items of at most UP_K cones of one cell (boxes in order
+        // grouped by (part, cell) into items of at most UP_K_CHOICES[i] cones of one cell (boxes in order
         // within a cell), and the other parent cones (per-box upward pass)
         std::vector<int32_t> leaf, up, glist;
-        std::vector<WorkItem> uitems;
+        std::vector<WorkItem> uitems[3];
         D.up_off.assign(D.levels + 1, 0);
         D.up_gl_off.assign(D.levels + 1, 0);
-        D.up_item_off.assign(D.levels + 1, 0);
+        for (int kc = 0; kc < 3; kc++)
+            D.up_item_off_k[kc].assign(D.levels + 1, 0);
         D.cone_lvl_off.assign(D.levels + 1, 0);
         for (int l = 0; l < D.levels; l++)
         {
@@ -1233,7 +1305,8 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             D.cone_lvl_off[l + 1] = P.cone_off[b1];
             D.up_off[l] = int64_t(up.size());
             D.up_gl_off[l] = int64_t(glist.size());
-            D.up_item_off[l] = int64_t(uitems.size());
+            for (int kc = 0; kc < 3; kc++)
+                D.up_item_off_k[kc][l] = int64_t(uitems[kc].size());
             const int64_t g0 = int64_t(glist.size());
             for (int64_t g = b0; g < b1; g++)
                 for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
@@ -1242,23 +1315,29 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             auto cu = [&](int32_t c) { return parts[part(c)]->cellmap[P.cone_cell[c]]; };
             std::stable_sort(glist.begin() + g0, glist.end(), [&](int32_t a, int32_t b)
                              { return part(a) != part(b) ? part(a) < part(b) : cu(a) < cu(b); });
-            for (int64_t i = g0; i < int64_t(glist.size());)
-            {
-                const int32_t pa = part(glist[i]), u = cu(glist[i]);
-                if (u < 0)
-                    throw std::runtime_error("upward table has no entry for an active cell");
-                int64_t j = i;
-                while (j < int64_t(glist.size()) && j - i < UP_K && part(glist[j]) == pa &&
-                       cu(glist[j]) == u)
-                    j++;
-                uitems.push_back({u, int32_t(i - g0), int32_t(j - i), pa});
-                i = j;
-            }
+            for (int kc = 0; kc < 3; kc++)
+                for (int64_t i = g0; i < int64_t(glist.size());)
+                {
+                    const int32_t pa = part(glist[i]), u = cu(glist[i]);
+                    if (u < 0)
+                        throw std::runtime_error("upward table has no entry for an active cell");
+                    int64_t j = i;
+                    while (j < int64_t(glist.size()) && j - i < UP_K_CHOICES[kc] &&
+                           part(glist[j]) == pa && cu(glist[j]) == u)
+                        j++;
+                    uitems[kc].push_back({u, int32_t(i - g0), int32_t(j - i), pa});
+                    i = j;
+                }
         }
         D.up_off[D.levels] = int64_t(up.size());
         D.up_gl_off[D.levels] = int64_t(glist.size());
-        D.up_item_off[D.levels] = int64_t(uitems.size());
-        D.up_items = dupload(uitems, tot);
+        for (int kc = 0; kc < 3; kc++)
+        {
+            D.up_item_off_k[kc][D.levels] = int64_t(uitems[kc].size());
+            D.up_items_k[kc] = dupload(uitems[kc], tot);
+        }
+        D.up_items = D.up_items_k[1];
+        D.up_item_off = D.up_item_off_k[1];
         D.up_glist = dupload(glist, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
@@ -1382,12 +1461,14 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
     try
     {
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const bool timed = std::getenv("IFGF_PHASE_TIMING") != nullptr;
+        const bool timed = g_force_timing || std::getenv("IFGF_PHASE_TIMING") != nullptr;
+        dev::set_xcd_log2(D.shape.xcd_log2);
+        set_xcd_log2_batched(D.shape.xcd_log2);
         // several columns: the batched pipeline (8 or 32 columns per pass) when the orders
         // have compiled kernels and the batch buffers fit
         if (nrhs >= 2 && batched_supported(D) && std::getenv("IFGF_NO_BATCH") == nullptr)
         {
-            int rb = nrhs <= 8 ? 8 : 32;
+            int rb = D.shape.rb > 0 ? D.shape.rb : (nrhs <= 8 ? 8 : 32);
             bool ok = ensure_batch(D, rb, !device_ptrs);
             if (!ok && rb == 32)
                 ok = ensure_batch(D, rb = 8, !device_ptrs);
@@ -1452,6 +1533,145 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
         return e.what();
     }
     return "";
+}
+
+std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
+{
+    if (sh.xcd_log2 < 0 || sh.xcd_log2 > 8)
+        return "launch shape: XCD run length must be 2^0 .. 2^8";
+    int kc = -1;
+    for (int i = 0; i < 3; i++)
+        if (UP_K_CHOICES[i] == sh.up_k)
+            kc = i;
+    if (kc < 0)
+        return "launch shape: upward cones per item must be 4, 8 or 16";
+    if (sh.rb != 0 && sh.rb != 8 && sh.rb != 32)
+        return "launch shape: batched columns per pass must be 8 or 32";
+    if (sh.cpb != 0)
+    {
+        const auto ch = transform_cpb_choices(order_class(D), D.prec == PREC_FP64 ? 8 : 4);
+        if (std::find(ch.begin(), ch.end(), sh.cpb) == ch.end())
+            return "launch shape: no transform kernel with that many cones per workgroup for "
+                   "these orders and this precision";
+    }
+    const bool changed = sh.xcd_log2 != D.shape.xcd_log2 || sh.cpb != D.shape.cpb ||
+                         sh.up_k != D.shape.up_k;
+    D.shape = sh;
+    D.up_items = D.up_items_k[kc];
+    D.up_item_off = D.up_item_off_k[kc];
+    if (changed && D.graph_exec)
+    {
+        // the captured kernels have the old shapes: capture again on the next graph mult
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.graph_exec));
+        D.graph_exec = nullptr;
+        if (D.graph)
+            (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
+        D.graph = nullptr;
+    }
+    return "";
+}
+
+std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best)
+{
+    double* w = nullptr;
+    double* o = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string err;
+    try
+    {
+        nrhs = std::max<int64_t>(1, nrhs);
+        reps = std::max(1, reps);
+        // synthetic densities: a fixed pseudo-random pattern (the timings do not depend on it)
+        std::vector<double> hw(2 * size_t(D.nsrc) * size_t(nrhs));
+        uint64_t x = 0x9e3779b97f4a7c15ull;
+        for (double& v : hw)
+        {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            v = double(x >> 11) * 0x1.0p-53 - 0.5;
+        }
+        HIP_OK(hipMalloc(&w, hw.size() * sizeof(double)));
+        HIP_OK(hipMalloc(&o, 2 * size_t(D.ntgt) * size_t(nrhs) * sizeof(double)));
+        HIP_OK(hipMemcpy(w, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        const bool batched = nrhs >= 2 && batched_supported(D);
+        // fastest of reps mults with the shape s (one untimed mult first): the phase's time
+        // from the phase events, or the whole mult's (phase < 0)
+        auto measure = [&](const LaunchShape& s, int phase) -> double
+        {
+            std::string e = device_set_shape(D, s);
+            if (!e.empty())
+                throw std::runtime_error(e);
+            double t_best = 1e300;
+            for (int r = -1; r < reps; r++)
+            {
+                g_force_timing = phase >= 0;
+                HIP_OK(hipEventRecord(e0, st));
+                e = device_mult(D, w, o, nrhs, true, st);
+                HIP_OK(hipEventRecord(e1, st));
+                HIP_OK(hipEventSynchronize(e1));
+                g_force_timing = false;
+                if (!e.empty())
+                    throw std::runtime_error(e);
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                const double t = phase >= 0 ? double(g_last.ms[phase]) : double(ms);
+                if (r >= 0)
+                    t_best = std::min(t_best, t);
+            }
+            return t_best;
+        };
+        LaunchShape cur = D.shape;
+        auto pick = [&](int LaunchShape::*field, const std::vector<int>& choices, int phase)
+        {
+            int arg = cur.*field;
+            double tb = 1e300;
+            for (int c : choices)
+            {
+                LaunchShape s = cur;
+                s.*field = c;
+                const double t = measure(s, phase);
+                if (t < tb)
+                {
+                    tb = t;
+                    arg = c;
+                }
+            }
+            cur.*field = arg;
+        };
+        pick(&LaunchShape::xcd_log2, {0, 2, 4, 6}, -1);
+        if (!batched)
+        {
+            const auto ch = transform_cpb_choices(order_class(D), D.prec == PREC_FP64 ? 8 : 4);
+            if (ch.size() > 1)
+                pick(&LaunchShape::cpb, ch, 3);
+        }
+        pick(&LaunchShape::up_k, {4, 8, 16}, batched ? -1 : 2);
+        if (batched && nrhs > 8)
+            pick(&LaunchShape::rb, {8, 32}, -1);
+        err = device_set_shape(D, cur);
+        best = cur;
+    }
+    catch (const std::exception& ex)
+    {
+        err = std::string("autotune: ") + ex.what();
+        g_force_timing = false;
+    }
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    if (st)
+        (void)hipStreamDestroy(st);
+    if (w)
+        (void)hipFree(w);
+    if (o)
+        (void)hipFree(o);
+    return err;
 }
 
 void device_last_timings(const DevicePlan&, double* out6)

@@ -59,6 +59,17 @@ struct WorkItem // a chunk of at most 64 consecutive target points of one target
     int32_t pad;
 };
 
+// Launch shapes of the mult's kernels: choices that change speed but not results, picked by
+// device_autotune (measured on the operator itself) or set by the caller.
+struct LaunchShape
+{
+    int xcd_log2 = 4; // work items dealt to the XCDs in runs of 2^xcd_log2 (0: plain order)
+    int cpb = 0;      // transform: cones per workgroup (0: the order class's default)
+    int up_k = 8;     // upward: parent cones of one cell per workgroup (4, 8 or 16)
+    int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
+};
+constexpr int UP_K_CHOICES[3] = {4, 8, 16};
+
 struct DevicePlan
 {
     int prec = PREC_MIXED;
@@ -97,7 +108,11 @@ struct DevicePlan
     int64_t* up_part_ent = nullptr;
     int64_t n_up_parts = 0;
 
-    WorkItem* up_items = nullptr;            // (cell, cones of that cell, part) items
+    // (cell, cones of that cell, part) items of at most UP_K_CHOICES[i] cones, one list per
+    // choice; up_items / up_item_off name the list of the current shape
+    WorkItem* up_items_k[3] = {nullptr, nullptr, nullptr};
+    std::vector<int64_t> up_item_off_k[3];
+    WorkItem* up_items = nullptr;
     std::vector<int64_t> up_item_off;        // per level, into up_items
     int32_t* up_glist = nullptr;             // the items' cones, level-major
     std::vector<int64_t> up_gl_off;          // per level, into up_glist
@@ -132,6 +147,8 @@ struct DevicePlan
     void* graph_exec = nullptr; // hipGraphExec_t
     void* cap_stream = nullptr; // hipStream_t of the capture
 
+    LaunchShape shape;
+
     int b_rb = 0;
     void *b_ws = nullptr, *b_res = nullptr, *b_V = nullptr, *b_F = nullptr;
     double *b_in = nullptr, *b_out = nullptr;
@@ -156,6 +173,14 @@ enum
 };
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
                         void* stream, int io = 0, bool graph = false);
+// Apply a launch shape (validated; an unknown choice is an error).  Drops a captured graph.
+std::string device_set_shape(DevicePlan& D, const LaunchShape& s);
+// Time the candidate shapes on this operator with synthetic densities (nrhs columns, each
+// candidate `reps` times, the fastest kept), one choice after another: the XCD run length on
+// the whole mult, then the transform's cones per workgroup and the upward cones per item on
+// their phases, then (nrhs >= 2) the batched columns per pass.  Leaves the best shape applied.
+std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best);
+void set_xcd_log2_batched(int lg);
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
 void device_last_timings(const DevicePlan& D, double* out6);
 bool device_available();

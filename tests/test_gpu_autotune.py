@@ -1,0 +1,72 @@
+"""Launch shapes and the autotuner: every shape gives the same results as the default one
+(shapes change the mapping of work to workgroups and XCDs, never the arithmetic), the tuner
+returns a valid shape and leaves it applied, and captured graphs follow a shape change."""
+import itertools
+
+import numpy as np
+import pytest
+
+import ifgf_amd
+from conftest import cdens, sphere, spheroid
+
+pytestmark = pytest.mark.gpu
+
+CPB = {("mixed", 6): [6, 2, 4, 8], ("fp64", 6): [3, 1, 2, 4], ("mixed", 8): [2, 1, 3],
+       ("fp64", 8): [1]}
+
+
+@pytest.mark.parametrize("prec,order", [("mixed", 6), ("fp64", 6), ("mixed", 8), ("fp64", 8)])
+def test_every_shape_gives_the_same_result(gpu, prec, order):
+    x = sphere(12000, 601)
+    w = cdens(12000, 602)
+    op = ifgf_amd.HelmholtzIfgfOperator(12.0, 64, order, 1, -1, precision=prec)
+    op.init(x.T, x.T)
+    ref = op.mult(w)
+    for lg, cpb, k in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16]):
+        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k)
+        print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k}", flush=True)
+        assert np.array_equal(op.mult(w), ref), (lg, cpb, k)
+
+
+def test_batched_shapes_and_autotune(gpu):
+    import torch
+    x = spheroid(20000, 603)
+    W = np.stack([cdens(20000, 604 + i) for i in range(20)], axis=1)
+    op = ifgf_amd.HelmholtzIfgfOperator(16.0, 64, 6, 1, -1)
+    op.init(x.T, x.T)
+    ref = op.mult(W)
+    for rb, k in itertools.product([8, 32], [4, 16]):
+        op.launch_shape = dict(batch_columns=rb, up_cones=k)
+        print(f"batched shape rb={rb} up_k={k}", flush=True)
+        U = op.mult(W)
+        assert np.array_equal(U, ref), (rb, k)
+    best = op.autotune(nrhs=20, reps=2)
+    print(f"batched autotune: {best}", flush=True)
+    assert best == op.launch_shape
+    assert best["batch_columns"] in (8, 32) and best["up_cones"] in (4, 8, 16)
+    Wd = torch.as_tensor(W, device="cuda:0")
+    assert np.array_equal(op.mult(Wd).cpu().numpy(), ref)
+
+
+def test_autotune_single_column_and_graph_recapture(gpu):
+    import torch
+    x = sphere(30000, 605)
+    w = cdens(30000, 606)
+    op = ifgf_amd.HelmholtzIfgfOperator(20.0, 64, 6, 1, -1)
+    op.init(x.T, x.T)
+    ref = op.mult(w)
+    wd = torch.as_tensor(w, device="cuda:0")
+    op.use_graph = True
+    g1 = op.mult(wd).cpu().numpy()
+    best = op.autotune(nrhs=1, reps=2)
+    print(f"single-column autotune: {best}", flush=True)
+    assert best == op.launch_shape
+    assert 0 <= best["xcd_run_log2"] <= 6 and best["transform_cones"] in CPB[("mixed", 6)]
+    # a different shape after the capture: the graph is captured again with the new kernels
+    op.launch_shape = dict(up_cones=4 if best["up_cones"] != 4 else 16, transform_cones=2)
+    g2 = op.mult(wd).cpu().numpy()
+    assert np.array_equal(g1, ref) and np.array_equal(g2, ref)
+    with pytest.raises(ifgf_amd.IfgfError):
+        op.launch_shape = dict(up_cones=5)
+    with pytest.raises(ifgf_amd.IfgfError):
+        op.launch_shape = dict(transform_cones=7)
