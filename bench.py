@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (no tuning after init)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device once (a plain copy kernel of "
                          "known size, for scaling profiler byte counters)")
@@ -157,6 +159,11 @@ def main():
     t_init = time.time() - t0
     work = op.plan_work()
     print(f"# init {t_init:.1f}s  {op.plan_counts()}", file=sys.stderr, flush=True)
+    # launch shapes: timed on this operator once after init (untimed, like init)
+    t0 = time.time()
+    shape = op.launch_shape if args.no_autotune else op.autotune(nrhs=1, reps=3)
+    t_tune = time.time() - t0
+    print(f"# launch shape {shape} (autotune {t_tune:.1f}s)", file=sys.stderr, flush=True)
     w_dev = torch.as_tensor(w, dtype=torch.complex128, device=dev)
     ms, u = time_mults(op, w_dev, args.steps, args.warmup)
     print(f"# mixed: {ms:.3f} ms/matvec", file=sys.stderr, flush=True)
@@ -190,6 +197,8 @@ def main():
         op64 = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
                                               precision="fp64")
         op64.init(x.T, x.T)
+        if not args.no_autotune:
+            op64.autotune(nrhs=1, reps=2)
         ms64, u64 = time_mults(op64, w_dev, max(2, args.steps // 3), 1)
         u64_np = u64.cpu().numpy()
         fp64 = {
@@ -223,8 +232,10 @@ def main():
         "traffic": traffic,
         "kernel": dom,
         "launches_per_mult": kc["launches_per_mult"] if kc else None,
-        "note": "compute-bound on the vector ALUs (MI355X FP32 vector peak = FP32 matrix peak = "
-                "157.3 TFLOP/s): algorithmic flops of the phase per mult / its device time "
+        "note": "compute-bound on the vector ALUs; peak = MI355X FP32 157.3 TFLOP/s (vector with "
+                "packed FP32, and matrix); the kernels are built without packed FP32 "
+                "instructions, so their vector ceiling is half of it: algorithmic flops of the "
+                "phase per mult / its device time "
                 "(= per launch / mean launch duration); traffic = HBM bytes per launch from "
                 "rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a plain copy (" +
                 (counters.get("source", "") if counters else "no counter profile") + ")",
@@ -265,6 +276,8 @@ def main():
         "rel_err_vs_direct": err,
         "rel_err_samples": int(len(idx)),
         "init_s": t_init,
+        "launch_shape": shape,
+        "autotune_s": t_tune,
         "fp64_form": fp64,
         "mixed_over_fp64": (fp64["ms_per_step"] / ms) if fp64 else None,
         "roofline": roofline,
