@@ -29,7 +29,7 @@ struct ifgf_operator
     // otherwise a transposed operator (sources and targets swapped) is built on first use
     bool same_points = false;
     bool merged = false; // several geometries (ifgf_merge)
-    bool host_tree = false; // octrees sorted and built on the host (IFGF_INIT_HOST_TREE)
+    bool host_plan = false; // octrees and cone marking on the host (IFGF_INIT_HOST_PLAN)
     ifgf_cutoff_fn cutoff = nullptr;
     void* cutoff_ctx = nullptr;
     std::unique_ptr<ifgf_operator> adj;
@@ -120,10 +120,11 @@ extern "C"
             std::function<bool(double)> cb;
             if (cutoff)
                 cb = [cutoff, cutoff_ctx](double d) { return cutoff(d, cutoff_ctx) != 0; };
-            op->host_tree = plan_only || (flags & IFGF_INIT_HOST_TREE) != 0;
-            std::string e = build_plan(op->plan, srcs, n_src, tgts, n_tgt, normals, op->leaf,
-                                       op->order, op->n_elem, cb, default_threads(),
-                                       op->host_tree ? TreeBuilder() : device_build_tree);
+            op->host_plan = plan_only || (flags & IFGF_INIT_HOST_PLAN) != 0;
+            std::string e = build_plan(
+                op->plan, srcs, n_src, tgts, n_tgt, normals, op->leaf, op->order, op->n_elem,
+                cb, default_threads(), op->host_plan ? TreeBuilder() : device_build_tree,
+                op->host_plan ? ConeMarker() : make_device_cone_marker());
             if (!e.empty())
                 return fail(op, IFGF_EINVAL, e);
             op->planned = true;
@@ -206,11 +207,12 @@ extern "C"
                         void* ctx = op->cutoff_ctx;
                         cb = [f, ctx](double d) { return f(d, ctx) != 0; };
                     }
-                    a->host_tree = op->host_tree;
-                    std::string e = build_plan(a->plan, ys.data(), T.n, xs.data(), S.n, nullptr,
-                                               a->leaf, a->order, a->n_elem, cb,
-                                               default_threads(),
-                                               a->host_tree ? TreeBuilder() : device_build_tree);
+                    a->host_plan = op->host_plan;
+                    std::string e = build_plan(
+                        a->plan, ys.data(), T.n, xs.data(), S.n, nullptr, a->leaf, a->order,
+                        a->n_elem, cb, default_threads(),
+                        a->host_plan ? TreeBuilder() : device_build_tree,
+                        a->host_plan ? ConeMarker() : make_device_cone_marker());
                     if (!e.empty())
                         return fail(op, IFGF_EINTERNAL, "adjoint plan: " + e);
                     a->planned = true;

@@ -188,4 +188,6 @@ bool device_available();
 // The source or target octree of init on the device (tree.hip): the same Tree as build_tree
 // (identical permutation, boxes and point ranges).  Returns "" or an error.
 std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T);
+// The cone marking of build_cones on the device (cones.hip): the same bitmaps as the host's.
+ConeMarker make_device_cone_marker();
 } // namespace ifgf

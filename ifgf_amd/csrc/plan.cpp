@@ -9,7 +9,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -18,6 +21,22 @@
 namespace ifgf
 {
 static const double PI = 3.14159265358979323846;
+
+// IFGF_PLAN_TIMING set: the wall time of each init step on stderr
+struct StepClock
+{
+    bool on = std::getenv("IFGF_PLAN_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void lap(const char* what)
+    {
+        if (!on)
+            return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "ifgf init: %-28s %8.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
 
 int default_threads()
 {
@@ -900,7 +919,7 @@ static void build_up_level(Plan& P, int l, const std::vector<double>& ho_nodes, 
     U.valid = true;
 }
 
-void build_cones(Plan& P, int nthreads)
+void build_cones(Plan& P, int nthreads, const ConeMarker& marker)
 {
     const Tree& S = P.S;
     const Tree& T = P.T;
@@ -966,8 +985,95 @@ void build_cones(Plan& P, int nthreads)
                 P.hc[3 * g + d] = (dhi[d] - dlo[d]) / double(nc[d]);
             }
         }
+        const auto t_a = std::chrono::steady_clock::now();
         if (level > 0)
             build_up_level(P, level - 1, ho_nodes, nthreads);
+        const auto t_b = std::chrono::steady_clock::now();
+        // activity bitmaps of the level's boxes (64-bit words; fine and coarse grids)
+        LevelBits lb;
+        lb.foff.assign(l1 - l0 + 1, 0);
+        lb.coff.assign(l1 - l0 + 1, 0);
+        for (int64_t q = 0; q < l1 - l0; q++)
+        {
+            const int64_t g = l0 + q;
+            const int64_t nfc = P.nf[3 * g] * P.nf[3 * g + 1] * P.nf[3 * g + 2];
+            const int64_t ncc = P.nc[3 * g] * P.nc[3 * g + 1] * P.nc[3 * g + 2];
+            lb.foff[q + 1] = lb.foff[q] + (nfc + 63) / 64;
+            lb.coff[q + 1] = lb.coff[q] + (ncc + 63) / 64;
+        }
+        lb.fbits.assign(size_t(lb.foff[l1 - l0]), 0);
+        lb.cbits.assign(size_t(lb.coff[l1 - l0]), 0);
+        if (marker)
+            marker(P, level, far_off, far_tgt, ho_nodes, lb);
+        else if (l1 > l0)
+            pfor(l1 - l0, nthreads,
+                 [&](int64_t q, int)
+                 {
+                     const int64_t g = l0 + q;
+                     const double H = S.side[g];
+                     const double* xc = &S.ctr[3 * g];
+                     const Grid fg = box_grid(P, g, 0), cg = box_grid(P, g, 1);
+                     uint64_t* fbits = &lb.fbits[lb.foff[q]];
+                     uint64_t* cbits = &lb.cbits[lb.coff[q]];
+                     auto mark_ids = [&](int64_t c0, int64_t c1)
+                     {
+                         if (c1 >= 0)
+                             cbits[c1 >> 6] |= uint64_t(1) << (c1 & 63);
+                         if (c0 >= 0)
+                             fbits[c0 >> 6] |= uint64_t(1) << (c0 & 63);
+                     };
+                     auto mark = [&](const double s[3]) { mark_ids(fg.cell(s), cg.cell(s)); };
+                     for (int64_t r = far_off[g]; r < far_off[g + 1]; r++)
+                     {
+                         const int32_t t = far_tgt[r];
+                         for (int64_t i = T.begin[t]; i < T.end[t]; i++)
+                         {
+                             double s[3];
+                             cart_to_interp(&T.pts[3 * i], xc, H, s);
+                             mark(s);
+                         }
+                     }
+                     if (P.parent_has_far[g])
+                     {
+                         const int32_t pg = S.parent[g];
+                         const UpLevel& U = P.up[level - 1];
+                         if (U.valid)
+                         {
+                             int oct = 0;
+                             while (S.child[8 * pg + oct] != int32_t(g))
+                                 oct++;
+                             for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
+                             {
+                                 const int64_t u = U.cellmap[P.cone_cell[c]];
+                                 for (size_t k = 0; k < n_ho; k++)
+                                 {
+                                     const UpEntry& E = U.entries[(size_t(u) * n_ho + k) * 8 + oct];
+                                     if (E.fine >= 0)
+                                         mark_ids(E.fine, E.coarse);
+                                 }
+                             }
+                         }
+                         else
+                         {
+                             const double pH = S.side[pg];
+                             const double* pxc = &S.ctr[3 * pg];
+                             const Grid pgrid = box_grid(P, pg, 1);
+                             for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
+                             {
+                                 const int64_t cell = P.cone_cell[c];
+                                 for (size_t k = 0; k < n_ho; k++)
+                                 {
+                                     double sp[3], y[3], s[3];
+                                     pgrid.point(cell, &ho_nodes[3 * k], sp);
+                                     interp_to_cart(sp, pxc, pH, y);
+                                     cart_to_interp(y, xc, H, s);
+                                     mark(s);
+                                 }
+                             }
+                         }
+                     }
+                 });
+        const auto t_m = std::chrono::steady_clock::now();
         struct Out
         {
             std::vector<int64_t> cells;
@@ -980,69 +1086,13 @@ void build_cones(Plan& P, int nthreads)
              [&](int64_t q, int)
              {
                  const int64_t g = l0 + q;
-                 const double H = S.side[g];
-                 const double* xc = &S.ctr[3 * g];
                  const Grid fg = box_grid(P, g, 0), cg = box_grid(P, g, 1);
-                 std::vector<uint64_t> fbits((fg.total() + 63) / 64, 0), cbits((cg.total() + 63) / 64, 0);
-                 auto mark_ids = [&](int64_t c0, int64_t c1)
-                 {
-                     if (c1 >= 0)
-                         cbits[c1 >> 6] |= uint64_t(1) << (c1 & 63);
-                     if (c0 >= 0)
-                         fbits[c0 >> 6] |= uint64_t(1) << (c0 & 63);
-                 };
-                 auto mark = [&](const double s[3]) { mark_ids(fg.cell(s), cg.cell(s)); };
-                 for (int64_t r = far_off[g]; r < far_off[g + 1]; r++)
-                 {
-                     const int32_t t = far_tgt[r];
-                     for (int64_t i = T.begin[t]; i < T.end[t]; i++)
-                     {
-                         double s[3];
-                         cart_to_interp(&T.pts[3 * i], xc, H, s);
-                         mark(s);
-                     }
-                 }
-                 if (P.parent_has_far[g])
-                 {
-                     const int32_t pg = S.parent[g];
-                     const UpLevel& U = P.up[level - 1];
-                     if (U.valid)
-                     {
-                         int oct = 0;
-                         while (S.child[8 * pg + oct] != int32_t(g))
-                             oct++;
-                         for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
-                         {
-                             const int64_t u = U.cellmap[P.cone_cell[c]];
-                             for (size_t k = 0; k < n_ho; k++)
-                             {
-                                 const UpEntry& E = U.entries[(size_t(u) * n_ho + k) * 8 + oct];
-                                 if (E.fine >= 0)
-                                     mark_ids(E.fine, E.coarse);
-                             }
-                         }
-                     }
-                     else
-                     {
-                         const double pH = S.side[pg];
-                         const double* pxc = &S.ctr[3 * pg];
-                         const Grid pgrid = box_grid(P, pg, 1);
-                         for (int64_t c = P.cone_off[pg]; c < P.cone_off[pg + 1]; c++)
-                         {
-                             const int64_t cell = P.cone_cell[c];
-                             for (size_t k = 0; k < n_ho; k++)
-                             {
-                                 double sp[3], y[3], s[3];
-                                 pgrid.point(cell, &ho_nodes[3 * k], sp);
-                                 interp_to_cart(sp, pxc, pH, y);
-                                 cart_to_interp(y, xc, H, s);
-                                 mark(s);
-                             }
-                         }
-                     }
-                 }
+                 const uint64_t* fbits = &lb.fbits[lb.foff[q]];
+                 const uint64_t* cbits = &lb.cbits[lb.coff[q]];
+                 const int64_t nfw = lb.foff[q + 1] - lb.foff[q];
+                 const int64_t ncw = lb.coff[q + 1] - lb.coff[q];
                  Out& o = out[q];
-                 for (int64_t w = 0; w < int64_t(cbits.size()); w++)
+                 for (int64_t w = 0; w < ncw; w++)
                      for (uint64_t bits = cbits[w]; bits; bits &= bits - 1)
                          o.cells.push_back(w * 64 + __builtin_ctzll(bits));
                  o.masks.assign(o.cells.size(), 0);
@@ -1067,7 +1117,7 @@ void build_cones(Plan& P, int nthreads)
                              }
                      o.masks[m] = mask;
                  }
-                 for (int64_t w = 0; w < int64_t(fbits.size()); w++)
+                 for (int64_t w = 0; w < nfw; w++)
                      for (uint64_t bits = fbits[w]; bits; bits &= bits - 1)
                      {
                          int64_t fc = w * 64 + __builtin_ctzll(bits);
@@ -1084,6 +1134,14 @@ void build_cones(Plan& P, int nthreads)
                              o.uncovered++;
                      }
              });
+        const auto t_c = std::chrono::steady_clock::now();
+        if (std::getenv("IFGF_PLAN_TIMING"))
+            std::fprintf(stderr,
+                         "ifgf init:   level %d: upward table %.1f ms, marking %.1f ms, "
+                         "cone lists %.1f ms\n",
+                         level, std::chrono::duration<double, std::milli>(t_b - t_a).count(),
+                         std::chrono::duration<double, std::milli>(t_m - t_b).count(),
+                         std::chrono::duration<double, std::milli>(t_c - t_m).count());
         for (int64_t q = 0; q < l1 - l0; q++)
         {
             const int64_t g = l0 + q;
@@ -1114,7 +1172,7 @@ void build_cones(Plan& P, int nthreads)
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
                        const std::function<bool(double)>& cutoff, int nthreads,
-                       const TreeBuilder& tree_builder)
+                       const TreeBuilder& tree_builder, const ConeMarker& cone_marker)
 {
     if (N <= 0 || M <= 0)
         return "empty point set";
@@ -1126,6 +1184,7 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
     P.base_els[1] = 2 * int64_t(n_elem);
     P.base_els[2] = 4 * int64_t(n_elem);
     int64_t max_leaf = leaf < 0 ? std::numeric_limits<int64_t>::max() : leaf;
+    StepClock clk;
     if (tree_builder)
     {
         std::string e = tree_builder(srcs, N, max_leaf, P.S);
@@ -1139,6 +1198,7 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
         build_tree(srcs, N, max_leaf, P.S, nthreads);
         build_tree(tgts, M, max_leaf, P.T, nthreads);
     }
+    clk.lap(tree_builder ? "octrees (device)" : "octrees (host)");
     for (int d = 0; d < 3; d++)
         if (!(P.S.hi[d] > P.S.lo[d]) || !(P.T.hi[d] > P.T.lo[d]))
             return "degenerate bounding box (all points share a coordinate plane at the origin)";
@@ -1151,6 +1211,7 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
                 P.normals[3 * i + d] = normals[3 * P.S.perm[i] + d];
     }
     build_interactions(P, cutoff);
+    clk.lap("interaction lists");
     if (P.K.tol > 0)
         estimate_orders(P, P.K.tol, P.base_order);
     else
@@ -1163,7 +1224,9 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
             return "interpolation order below 1 (the fine order of this kernel is order-3)";
         P.factor[d] = P.K.fine_factor();
     }
-    build_cones(P, nthreads);
+    clk.lap("orders");
+    build_cones(P, nthreads, cone_marker);
+    clk.lap(cone_marker ? "cones (device marking)" : "cones (host marking)");
     return "";
 }
 // ---------------------------------------------------------------------------------------

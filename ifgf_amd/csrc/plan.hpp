@@ -190,13 +190,30 @@ using TreeBuilder = std::function<std::string(const double* pts, int64_t n, int6
                                               Tree& T)>;
 void build_interactions(Plan& P, const std::function<bool(double)>& cutoff);
 int estimate_orders(Plan& P, double tol, int out[3]); // ifgfoperator.hpp:108-244
-void build_cones(Plan& P, int nthreads);
+// Cone activity of one source level: for box q of the level (global id level_off + q) the
+// fine-grid and coarse-grid bitmaps (bit = cell id) in 64-bit words fbits[foff[q] ..
+// foff[q + 1]) and cbits[coff[q] .. coff[q + 1]), zeroed by build_cones.  A cell is active
+// when a far target of the box, or a high-order node of an active coarse cone of its parent,
+// lies in it (octree.hpp:487-548, 620-700).  The marker (when set) fills them; build_cones
+// otherwise marks on the host.  Then each box's active coarse cones (ascending cell), fine
+// sub-cone masks and fine cell lists follow from the bitmaps.
+struct LevelBits
+{
+    std::vector<int64_t> foff, coff;
+    std::vector<uint64_t> fbits, cbits;
+};
+using ConeMarker = std::function<void(const Plan& P, int level,
+                                      const std::vector<int64_t>& far_off,
+                                      const std::vector<int32_t>& far_tgt,
+                                      const std::vector<double>& ho_nodes, LevelBits& bits)>;
+void build_cones(Plan& P, int nthreads, const ConeMarker& marker = nullptr);
 
 // Build everything init() needs (ifgfoperator.hpp:65-106).  Returns "" or an error.
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
                        const std::function<bool(double)>& cutoff, int nthreads,
-                       const TreeBuilder& tree_builder = nullptr);
+                       const TreeBuilder& tree_builder = nullptr,
+                       const ConeMarker& cone_marker = nullptr);
 
 // One plan for several planned geometries of one kernel family and one set of orders: the
 // boxes of every level are those of part 0, then part 1, ...; points, cones, coarse cells and

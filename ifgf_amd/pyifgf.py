@@ -16,6 +16,8 @@ step in FP64).  mult also accepts a 2-D array (N, nrhs) and torch tensors on the
 """
 import ctypes
 
+import sys
+
 import numpy as np
 
 from . import _lib
@@ -61,15 +63,18 @@ class _IfgfOperator:
         # single-column mults of device tensors replay the mult's kernels as a HIP graph
         self.use_graph = False
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         h = getattr(self, "_h", None)
-        if h:
-            lib().ifgf_destroy(h)
-            self._h = None
+        if not h:
+            return
+        self._h = None
+        if _finalizing():
+            return  # interpreter exit: the process's device memory goes with it
+        lib().ifgf_destroy(h)
 
     # IfgfOperator::init (ifgfoperator.hpp:65-106)
     def init(self, srcs, targets, cutOff=None, normals=None, *, plan_only=False,
-             host_tree=False):
+             host_plan=False):
         s = _points(srcs, "srcs")
         t = _points(targets, "targets")
         n = None if normals is None else _points(normals, "normals")
@@ -79,7 +84,7 @@ class _IfgfOperator:
         rc = lib().ifgf_init(self._h, s.ctypes.data, s.shape[0], t.ctypes.data, t.shape[0],
                              None if n is None else n.ctypes.data, cb, None,
                              (_lib.INIT_PLAN_ONLY if plan_only else 0)
-                             | (_lib.INIT_HOST_TREE if host_tree else 0))
+                             | (_lib.INIT_HOST_PLAN if host_plan else 0))
         check(rc, self._h, "init")
         self._n, self._m = s.shape[0], t.shape[0]
 

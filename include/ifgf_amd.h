@@ -38,8 +38,9 @@ extern "C" {
 
 /* ifgf_init flags */
 #define IFGF_INIT_PLAN_ONLY 1 /* build the host plan only (introspection); mult is refused */
-#define IFGF_INIT_HOST_TREE 2 /* sort the points and build the octrees on the host (by default
-                                 a device init builds them on the GPU; both give the same tree) */
+#define IFGF_INIT_HOST_PLAN 2 /* sort the points, build the octrees and mark the active cones
+                                 with the host code (by default a device init does these on
+                                 the GPU; both give the same plan) */
 
 /* ifgf_mult flags */
 #define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */

@@ -96,7 +96,7 @@ def test_device_tree_matches_host_tree(gpu, case):
     dev = ifgf_amd.HelmholtzIfgfOperator(6.0, leaf, 6, 1, -1)
     dev.init(x.T, y.T)
     host = ifgf_amd.HelmholtzIfgfOperator(6.0, leaf, 6, 1, -1)
-    host.init(x.T, y.T, host_tree=True)
+    host.init(x.T, y.T, host_plan=True)
     _plans_equal(dev, host)
     w = cdens(len(x), 511)
     a, b = dev.mult(w), host.mult(w)
