@@ -28,7 +28,7 @@ def build(verbose=True):
     for src in ("plan.cpp", "capi.cpp"):
         jobs.append((["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src)],
                      os.path.join(BUILD, src + ".o")))
-    for src in ("device.hip", "batched.hip", "tree.hip", "cones.hip"):
+    for src in ("device.hip", "batched.hip", "tree.hip", "cones.hip", "pairs.hip"):
         jobs.append(([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src)],
                      os.path.join(BUILD, src + ".o")))
     procs = []

@@ -121,10 +121,10 @@ extern "C"
             if (cutoff)
                 cb = [cutoff, cutoff_ctx](double d) { return cutoff(d, cutoff_ctx) != 0; };
             op->host_plan = plan_only || (flags & IFGF_INIT_HOST_PLAN) != 0;
-            std::string e = build_plan(
-                op->plan, srcs, n_src, tgts, n_tgt, normals, op->leaf, op->order, op->n_elem,
-                cb, default_threads(), op->host_plan ? TreeBuilder() : device_build_tree,
-                op->host_plan ? ConeMarker() : make_device_cone_marker());
+            std::string e =
+                build_plan(op->plan, srcs, n_src, tgts, n_tgt, normals, op->leaf, op->order,
+                           op->n_elem, cb, default_threads(),
+                           op->host_plan ? PlanBuilders() : device_plan_builders());
             if (!e.empty())
                 return fail(op, IFGF_EINVAL, e);
             op->planned = true;
@@ -208,11 +208,10 @@ extern "C"
                         cb = [f, ctx](double d) { return f(d, ctx) != 0; };
                     }
                     a->host_plan = op->host_plan;
-                    std::string e = build_plan(
-                        a->plan, ys.data(), T.n, xs.data(), S.n, nullptr, a->leaf, a->order,
-                        a->n_elem, cb, default_threads(),
-                        a->host_plan ? TreeBuilder() : device_build_tree,
-                        a->host_plan ? ConeMarker() : make_device_cone_marker());
+                    std::string e =
+                        build_plan(a->plan, ys.data(), T.n, xs.data(), S.n, nullptr, a->leaf,
+                                   a->order, a->n_elem, cb, default_threads(),
+                                   a->host_plan ? PlanBuilders() : device_plan_builders());
                     if (!e.empty())
                         return fail(op, IFGF_EINTERNAL, "adjoint plan: " + e);
                     a->planned = true;

@@ -190,4 +190,15 @@ bool device_available();
 std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T);
 // The cone marking of build_cones on the device (cones.hip): the same bitmaps as the host's.
 ConeMarker make_device_cone_marker();
+// The interaction lists on the device (pairs.hip): the same sorted lists as build_interactions.
+std::string device_build_interactions(Plan& P);
+// all three device builders
+inline PlanBuilders device_plan_builders()
+{
+    PlanBuilders b;
+    b.tree = device_build_tree;
+    b.pairs = device_build_interactions;
+    b.cones = make_device_cone_marker();
+    return b;
+}
 } // namespace ifgf

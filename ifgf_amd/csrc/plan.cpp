@@ -500,7 +500,12 @@ void build_interactions(Plan& P, const std::function<bool(double)>& cutoff)
     { return a.tgt != b.tgt ? a.tgt < b.tgt : a.src < b.src; };
     std::sort(P.far.begin(), P.far.end(), by_tgt);
     std::sort(P.near.begin(), P.near.end(), by_tgt);
+    interaction_flags(P);
+}
 
+void interaction_flags(Plan& P)
+{
+    const Tree& S = P.S;
     const int64_t nb = S.num_boxes();
     std::vector<int64_t> nfar(nb, 0);
     for (const Pair& p : P.far)
@@ -1172,8 +1177,10 @@ void build_cones(Plan& P, int nthreads, const ConeMarker& marker)
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
                        const std::function<bool(double)>& cutoff, int nthreads,
-                       const TreeBuilder& tree_builder, const ConeMarker& cone_marker)
+                       const PlanBuilders& dev)
 {
+    const TreeBuilder& tree_builder = dev.tree;
+    const ConeMarker& cone_marker = dev.cones;
     if (N <= 0 || M <= 0)
         return "empty point set";
     if (n_elem <= 0)
@@ -1210,8 +1217,18 @@ std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgt
             for (int d = 0; d < 3; d++)
                 P.normals[3 * i + d] = normals[3 * P.S.perm[i] + d];
     }
-    build_interactions(P, cutoff);
-    clk.lap("interaction lists");
+    if (dev.pairs && !cutoff)
+    {
+        std::string e = dev.pairs(P);
+        if (!e.empty())
+            return e;
+        clk.lap("interaction lists (device)");
+    }
+    else
+    {
+        build_interactions(P, cutoff);
+        clk.lap("interaction lists (host)");
+    }
     if (P.K.tol > 0)
         estimate_orders(P, P.K.tol, P.base_order);
     else

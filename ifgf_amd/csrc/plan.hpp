@@ -189,6 +189,11 @@ void build_tree(const double* pts, int64_t n, int64_t max_leaf, Tree& T, int nth
 using TreeBuilder = std::function<std::string(const double* pts, int64_t n, int64_t max_leaf,
                                               Tree& T)>;
 void build_interactions(Plan& P, const std::function<bool(double)>& cutoff);
+// after the far and near lists are sorted by (target, source): has_far_anc, parent_has_far
+void interaction_flags(Plan& P);
+// builds P.far and P.near sorted by (target, source) and calls interaction_flags (the device
+// builder; used when no cutOff callback is set).  Returns "" or an error.
+using PairBuilder = std::function<std::string(Plan& P)>;
 int estimate_orders(Plan& P, double tol, int out[3]); // ifgfoperator.hpp:108-244
 // Cone activity of one source level: for box q of the level (global id level_off + q) the
 // fine-grid and coarse-grid bitmaps (bit = cell id) in 64-bit words fbits[foff[q] ..
@@ -208,12 +213,19 @@ using ConeMarker = std::function<void(const Plan& P, int level,
                                       const std::vector<double>& ho_nodes, LevelBits& bits)>;
 void build_cones(Plan& P, int nthreads, const ConeMarker& marker = nullptr);
 
+// the device builders of init (unset members: the host code)
+struct PlanBuilders
+{
+    TreeBuilder tree;
+    PairBuilder pairs;
+    ConeMarker cones;
+};
+
 // Build everything init() needs (ifgfoperator.hpp:65-106).  Returns "" or an error.
 std::string build_plan(Plan& P, const double* srcs, int64_t N, const double* tgts, int64_t M,
                        const double* normals, int64_t leaf, int order, int n_elem,
                        const std::function<bool(double)>& cutoff, int nthreads,
-                       const TreeBuilder& tree_builder = nullptr,
-                       const ConeMarker& cone_marker = nullptr);
+                       const PlanBuilders& dev = PlanBuilders());
 
 // One plan for several planned geometries of one kernel family and one set of orders: the
 // boxes of every level are those of part 0, then part 1, ...; points, cones, coarse cells and

@@ -4,9 +4,9 @@
 //   k_minmax           bounding box (first occurrence wins among equal extremes, as the
 //                      sequential std::min / std::max scan of the host does)
 //   k_keys             126-bit Morton keys, bit for bit the host morton_key
-//   k_rs_hist / k_rs_scan / k_rs_scatter
-//                      hand-written LSD radix sort of (key hi, key lo, index): 16 stable
-//                      passes of 8-bit digits; ties of the whole key keep index order
+//   radix::sort        hand-written LSD radix sort (radix_sort.hpp) of (key hi, key lo,
+//                      index) records: 16 stable passes of 8-bit digits; ties of the whole
+//                      key keep index order
 //   k_ties             each run of equal keys ordered by the exact comparison (stable
 //                      insertion sort = the host's stable_sort), runs up to TIE_RUN_MAX
 //   k_gather_pts       the Morton-sorted points
@@ -27,6 +27,7 @@
 
 #include "device.hpp"
 #include "plan.hpp"
+#include "radix_sort.hpp"
 
 #pragma clang fp contract(off)
 
@@ -44,10 +45,6 @@ namespace
 
 constexpr int KEY_BITS = 40;
 constexpr int MM_THREADS = 256;
-constexpr int RS_THREADS = 256; // four waves
-constexpr int RS_ITEMS = 8;     // keys per lane
-constexpr int RS_WAVE_SEG = 64 * RS_ITEMS;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_PASSES = 16; // 8-bit digits over 128 bits (the top two are zero)
 constexpr int64_t TIE_RUN_MAX = 1024;
 constexpr int SPLIT_THREADS = 1024;
@@ -56,6 +53,23 @@ constexpr int SPLIT_PER = 4; // points per thread per chunk
 struct Norm
 {
     double ctr[3], diam[3];
+};
+
+// a point's Morton key and caller index (the radix sort's record)
+struct KeyRec
+{
+    uint64_t hi, lo;
+    int32_t id, pad;
+};
+
+// digit of pass p: byte p of lo for p < 8, byte p - 8 of hi after
+struct KeyDigit
+{
+    int use_hi, shift;
+    __device__ __forceinline__ uint32_t operator()(const KeyRec& r) const
+    {
+        return uint32_t((use_hi ? r.hi : r.lo) >> shift) & 255u;
+    }
 };
 
 // ---------------------------------------------------------------------------------------
@@ -228,185 +242,48 @@ __device__ bool dev_less_exact(const double* px, const double* qx, const Norm& n
 }
 
 __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ pts, int64_t n, Norm nm,
-                                             uint64_t* __restrict__ khi,
-                                             uint64_t* __restrict__ klo,
-                                             int32_t* __restrict__ idx)
+                                             KeyRec* __restrict__ keys)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n)
         return;
     const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
-    uint64_t h, l;
-    dev_key(p, nm, h, l);
-    khi[i] = h;
-    klo[i] = l;
-    idx[i] = int32_t(i);
-}
-
-// ---------------------------------------------------------------------------------------
-// LSD radix sort
-// ---------------------------------------------------------------------------------------
-// per tile: digit counts, stored digit-major (counts[d * nblk + tile])
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ word,
-                                                       int64_t n, int shift,
-                                                       uint32_t* __restrict__ counts, int nblk)
-{
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t base = int64_t(blockIdx.x) * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; r++)
-    {
-        const int64_t i = base + int64_t(r) * RS_THREADS + threadIdx.x;
-        if (i < n)
-            atomicAdd(&h[uint32_t(word[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    counts[int64_t(threadIdx.x) * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// exclusive prefix sum of m counts in place (one workgroup; a contiguous chunk per thread)
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ c, int64_t m)
-{
-    __shared__ uint32_t s[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (m + 1023) / 1024;
-    const int64_t a = std::min<int64_t>(m, int64_t(t) * per), b = std::min<int64_t>(m, a + per);
-    uint32_t sum = 0;
-    for (int64_t i = a; i < b; i++)
-        sum += c[i];
-    s[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1)
-    {
-        const uint32_t v = t >= off ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = s[t] - sum;
-    for (int64_t i = a; i < b; i++)
-    {
-        const uint32_t v = c[i];
-        c[i] = run;
-        run += v;
-    }
-}
-
-// stable scatter of one tile: wave w owns keys [w * RS_WAVE_SEG, (w + 1) * RS_WAVE_SEG) of the
-// tile, taken 64 at a time in order; a key's rank among equal digits of its wave is the
-// wave's running count for the digit plus the lanes below it with the same digit
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
-    const uint64_t* __restrict__ hi_in, const uint64_t* __restrict__ lo_in,
-    const int32_t* __restrict__ id_in, uint64_t* __restrict__ hi_out,
-    uint64_t* __restrict__ lo_out, int32_t* __restrict__ id_out, int64_t n, int use_hi,
-    int shift, const uint32_t* __restrict__ offs, int nblk)
-{
-    __shared__ uint32_t cnt[RS_THREADS / 64][256];
-    __shared__ uint32_t gofs[256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < (RS_THREADS / 64) * 256; i += RS_THREADS)
-        (&cnt[0][0])[i] = 0;
-    gofs[threadIdx.x] = offs[int64_t(threadIdx.x) * nblk + blockIdx.x];
-    __syncthreads();
-    const int64_t seg = int64_t(blockIdx.x) * RS_TILE + int64_t(w) * RS_WAVE_SEG;
-    const uint64_t below_me = (uint64_t(1) << lane) - 1;
-    uint64_t kh[RS_ITEMS], kl[RS_ITEMS];
-    int32_t kid[RS_ITEMS];
-    uint32_t dg[RS_ITEMS], rk[RS_ITEMS];
-#pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++)
-    {
-        const int64_t i = seg + int64_t(r) * 64 + lane;
-        const bool valid = i < n;
-        uint32_t d = 0;
-        kh[r] = 0;
-        kl[r] = 0;
-        kid[r] = 0;
-        if (valid)
-        {
-            kh[r] = hi_in[i];
-            kl[r] = lo_in[i];
-            kid[r] = id_in[i];
-            d = uint32_t((use_hi ? kh[r] : kl[r]) >> shift) & 255u;
-        }
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++)
-        {
-            const bool set = (d >> b) & 1u;
-            const uint64_t m = __ballot(set);
-            peers &= set ? m : ~m;
-        }
-        dg[r] = d;
-        rk[r] = 0;
-        if (valid)
-        {
-            const uint32_t below = uint32_t(__popcll(peers & below_me));
-            const uint32_t base = cnt[w][d];
-            rk[r] = base + below;
-            if (below == 0)
-                cnt[w][d] = base + uint32_t(__popcll(peers));
-        }
-    }
-    __syncthreads();
-    {
-        const int d = threadIdx.x;
-        uint32_t s = 0;
-        for (int v = 0; v < RS_THREADS / 64; v++)
-        {
-            const uint32_t t = cnt[v][d];
-            cnt[v][d] = s;
-            s += t;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++)
-    {
-        const int64_t i = seg + int64_t(r) * 64 + lane;
-        const int64_t pos = int64_t(gofs[dg[r]]) + cnt[w][dg[r]] + rk[r];
-        if (i < n && pos < n)
-        {
-            hi_out[pos] = kh[r];
-            lo_out[pos] = kl[r];
-            id_out[pos] = kid[r];
-        }
-    }
+    KeyRec k;
+    dev_key(p, nm, k.hi, k.lo);
+    k.id = int32_t(i);
+    k.pad = 0;
+    keys[i] = k;
 }
 
 // the sorted indices form a permutation of 0..n-1 (seen[] starts at zero); bad is set otherwise
-__global__ __launch_bounds__(256) void k_check_perm(const int32_t* __restrict__ idx, int64_t n,
+__global__ __launch_bounds__(256) void k_check_perm(const KeyRec* __restrict__ keys, int64_t n,
                                                    uint32_t* __restrict__ seen,
                                                    int* __restrict__ bad)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n)
         return;
-    const int32_t s = idx[i];
+    const int32_t s = keys[i].id;
     if (s < 0 || int64_t(s) >= n || atomicAdd(&seen[s], 1u) != 0)
         atomicOr(bad, 1);
 }
 
-__device__ __forceinline__ bool same_key(const uint64_t* hi, const uint64_t* lo, int64_t a,
-                                         int64_t b)
+__device__ __forceinline__ bool same_key(const KeyRec* k, int64_t a, int64_t b)
 {
-    return hi[a] == hi[b] && lo[a] == lo[b];
+    return k[a].hi == k[b].hi && k[a].lo == k[b].lo;
 }
 
 // one thread per run of equal keys (at its first element): stable insertion sort of the run
 // by the exact comparison; runs longer than TIE_RUN_MAX are left to the host (flag)
-__global__ __launch_bounds__(256) void k_ties(const uint64_t* __restrict__ hi,
-                                             const uint64_t* __restrict__ lo,
-                                             int32_t* __restrict__ idx, int64_t n,
+__global__ __launch_bounds__(256) void k_ties(KeyRec* __restrict__ keys, int64_t n,
                                              const double* __restrict__ pts, Norm nm,
                                              int* __restrict__ long_run)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i + 1 >= n || !same_key(hi, lo, i, i + 1) || (i > 0 && same_key(hi, lo, i - 1, i)))
+    if (i + 1 >= n || !same_key(keys, i, i + 1) || (i > 0 && same_key(keys, i - 1, i)))
         return;
     int64_t j = i + 1;
-    while (j < n && same_key(hi, lo, i, j))
+    while (j < n && same_key(keys, i, j))
         j++;
     if (j - i > TIE_RUN_MAX)
     {
@@ -415,25 +292,26 @@ __global__ __launch_bounds__(256) void k_ties(const uint64_t* __restrict__ hi,
     }
     for (int64_t a = i + 1; a < j; a++)
     {
-        const int32_t x = idx[a];
+        const int32_t x = keys[a].id;
         int64_t b = a;
-        while (b > i && dev_less_exact(pts + 3 * int64_t(x), pts + 3 * int64_t(idx[b - 1]), nm))
+        while (b > i &&
+               dev_less_exact(pts + 3 * int64_t(x), pts + 3 * int64_t(keys[b - 1].id), nm))
         {
-            idx[b] = idx[b - 1];
+            keys[b].id = keys[b - 1].id;
             b--;
         }
-        idx[b] = x;
+        keys[b].id = x;
     }
 }
 
 __global__ __launch_bounds__(256) void k_gather_pts(const double* __restrict__ pts,
-                                                   const int32_t* __restrict__ idx, int64_t n,
+                                                   const KeyRec* __restrict__ keys, int64_t n,
                                                    double* __restrict__ spts)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n)
         return;
-    const int64_t s = idx[i];
+    const int64_t s = keys[i].id;
     for (int d = 0; d < 3; d++)
         spts[3 * i + d] = pts[3 * s + d];
 }
@@ -565,7 +443,7 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
             return "device tree: empty point set";
         if (n > int64_t(INT32_MAX))
             return "device tree: more than 2^31 - 1 points";
-        DBuf d_pts, d_spts, pv, pi, hi[2], lo[2], id[2], counts, flag;
+        DBuf d_pts, d_spts, pv, pi, counts, flag;
         h2d(d_pts, pts, size_t(3 * n));
 
         // bounding box: per-block first extremes, finished here with the same rule
@@ -606,32 +484,16 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
         tree_root(T, n, mn, mx, nm.ctr, nm.diam);
 
         // keys and the radix sort
-        for (int s = 0; s < 2; s++)
-        {
-            hi[s].reserve(sizeof(uint64_t) * n);
-            lo[s].reserve(sizeof(uint64_t) * n);
-            id[s].reserve(sizeof(int32_t) * n);
-        }
-        k_keys<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), n, nm, hi[0].as<uint64_t>(),
-                                            lo[0].as<uint64_t>(), id[0].as<int32_t>());
+        DBuf keys[2];
+        for (int b2 = 0; b2 < 2; b2++)
+            keys[b2].reserve(sizeof(KeyRec) * n);
+        k_keys<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), n, nm, keys[0].as<KeyRec>());
         TREE_HIP_OK(hipGetLastError());
-        const int nblk = blocks_for(n, RS_TILE);
-        counts.reserve(sizeof(uint32_t) * 256 * size_t(nblk));
-        int cur = 0;
-        for (int pass = 0; pass < RS_PASSES; pass++)
-        {
-            const int use_hi = pass >= 8;
-            const int shift = (pass & 7) * 8;
-            const uint64_t* word = use_hi ? hi[cur].as<uint64_t>() : lo[cur].as<uint64_t>();
-            k_rs_hist<<<nblk, RS_THREADS>>>(word, n, shift, counts.as<uint32_t>(), nblk);
-            k_rs_scan<<<1, 1024>>>(counts.as<uint32_t>(), int64_t(256) * nblk);
-            k_rs_scatter<<<nblk, RS_THREADS>>>(
-                hi[cur].as<uint64_t>(), lo[cur].as<uint64_t>(), id[cur].as<int32_t>(),
-                hi[1 - cur].as<uint64_t>(), lo[1 - cur].as<uint64_t>(), id[1 - cur].as<int32_t>(),
-                n, use_hi, shift, counts.as<uint32_t>(), nblk);
-            TREE_HIP_OK(hipGetLastError());
-            cur = 1 - cur;
-        }
+        counts.reserve(sizeof(uint32_t) * radix::counts_needed(n));
+        KeyRec* sorted = radix::sort(
+            keys[0].as<KeyRec>(), keys[1].as<KeyRec>(), n, RS_PASSES,
+            [](int pass) { return KeyDigit{pass >= 8 ? 1 : 0, (pass & 7) * 8}; },
+            counts.as<uint32_t>(), nullptr);
         flag.reserve(sizeof(int));
         TREE_HIP_OK(hipMemset(flag.p, 0, sizeof(int)));
         {
@@ -639,59 +501,52 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
             DBuf seen;
             seen.reserve(sizeof(uint32_t) * n);
             TREE_HIP_OK(hipMemset(seen.p, 0, sizeof(uint32_t) * n));
-            k_check_perm<<<blocks_for(n, 256), 256>>>(id[cur].as<int32_t>(), n,
-                                                      seen.as<uint32_t>(), flag.as<int>());
+            k_check_perm<<<blocks_for(n, 256), 256>>>(sorted, n, seen.as<uint32_t>(),
+                                                      flag.as<int>());
             TREE_HIP_OK(hipGetLastError());
             int bad = 0;
             TREE_HIP_OK(hipMemcpy(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost));
             if (bad)
                 return "device tree: the radix sort did not produce a permutation";
         }
-        k_ties<<<blocks_for(n, 256), 256>>>(hi[cur].as<uint64_t>(), lo[cur].as<uint64_t>(),
-                                            id[cur].as<int32_t>(), n, d_pts.as<double>(), nm,
-                                            flag.as<int>());
+        k_ties<<<blocks_for(n, 256), 256>>>(sorted, n, d_pts.as<double>(), nm, flag.as<int>());
         TREE_HIP_OK(hipGetLastError());
         int long_run = 0;
         TREE_HIP_OK(hipMemcpy(&long_run, flag.p, sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<int32_t> perm32(n);
-        TREE_HIP_OK(hipMemcpy(perm32.data(), id[cur].p, sizeof(int32_t) * n,
-                              hipMemcpyDeviceToHost));
+        std::vector<KeyRec> hk(n);
+        TREE_HIP_OK(hipMemcpy(hk.data(), sorted, sizeof(KeyRec) * n, hipMemcpyDeviceToHost));
         if (long_run)
         {
             // runs of more than TIE_RUN_MAX equal keys (coincident points): the host's stable
             // sort of each such run
-            std::vector<uint64_t> khi(n), klo(n);
-            TREE_HIP_OK(hipMemcpy(khi.data(), hi[cur].p, sizeof(uint64_t) * n,
-                                  hipMemcpyDeviceToHost));
-            TREE_HIP_OK(hipMemcpy(klo.data(), lo[cur].p, sizeof(uint64_t) * n,
-                                  hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < n;)
             {
                 int64_t j = i + 1;
-                while (j < n && khi[j] == khi[i] && klo[j] == klo[i])
+                while (j < n && hk[j].hi == hk[i].hi && hk[j].lo == hk[i].lo)
                     j++;
                 if (j - i > TIE_RUN_MAX)
-                    std::stable_sort(perm32.begin() + i, perm32.begin() + j,
-                                     [&](int32_t x, int32_t y)
+                    std::stable_sort(hk.begin() + i, hk.begin() + j,
+                                     [&](const KeyRec& x, const KeyRec& y)
                                      {
-                                         return morton_less_exact(pts + 3 * int64_t(x),
-                                                                  pts + 3 * int64_t(y), nm.ctr,
+                                         return morton_less_exact(pts + 3 * int64_t(x.id),
+                                                                  pts + 3 * int64_t(y.id), nm.ctr,
                                                                   nm.diam);
                                      });
                 i = j;
             }
-            h2d(id[cur], perm32.data(), size_t(n));
-        }
-        for (int s = 0; s < 2; s++)
-        {
-            hi[s].release();
-            lo[s].release();
+            TREE_HIP_OK(hipMemcpy(sorted, hk.data(), sizeof(KeyRec) * n, hipMemcpyHostToDevice));
         }
         d_spts.reserve(sizeof(double) * 3 * n);
-        k_gather_pts<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), id[cur].as<int32_t>(), n,
+        k_gather_pts<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), sorted, n,
                                                   d_spts.as<double>());
         TREE_HIP_OK(hipGetLastError());
-        T.perm.assign(perm32.begin(), perm32.end());
+        T.perm.resize(n);
+        for (int64_t i = 0; i < n; i++)
+            T.perm[i] = hk[i].id;
+        hk.clear();
+        hk.shrink_to_fit();
+        for (int b2 = 0; b2 < 2; b2++)
+            keys[b2].release();
         T.pts.resize(3 * size_t(n));
         TREE_HIP_OK(hipMemcpy(T.pts.data(), d_spts.p, sizeof(double) * 3 * n,
                               hipMemcpyDeviceToHost));
