@@ -395,14 +395,11 @@ __global__ __launch_bounds__(256) void k_transform_b(int64_t c0, int64_t c1,
     constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
     constexpr int YS = (M0 * H1 * H2 > M0 * M1 * M2) ? M0 * H1 * H2 : M0 * M1 * M2;
     constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
-    __shared__ R Ws[NW0 + NW1 + NW2];
     __shared__ Cx<R> X[NI * XS];
     __shared__ Cx<R> Y[NI * YS];
     __shared__ unsigned char subl[32];
     __shared__ int nsub;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NW0 + NW1 + NW2; i += 256)
-        Ws[i] = W[i];
     const int64_t inst0 = xcd_block() * NI; // first (cone, column) instance
     const int64_t cone = c0 + inst0 / RB;
     const int r0 = int(inst0 % RB);
@@ -423,9 +420,9 @@ __global__ __launch_bounds__(256) void k_transform_b(int64_t c0, int64_t c1,
         nsub = n;
     }
     __syncthreads();
-    const R* W0 = Ws;
-    const R* W1 = Ws + NW0;
-    const R* W2 = Ws + NW0 + NW1;
+    const R* W0 = W; // scalar loads at compile-time offsets (see k_transform_t)
+    const R* W1 = W + NW0;
+    const R* W2 = W + NW0 + NW1;
     for (int e = tid; e < NI * H1 * H2; e += 256)
     {
         const int k = e / (H1 * H2), p = e - k * (H1 * H2);

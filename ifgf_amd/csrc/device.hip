@@ -354,14 +354,11 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
     constexpr int YS = (M0 * H1 * H2 > M0 * M1 * M2) ? M0 * H1 * H2 : M0 * M1 * M2;
     constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
-    __shared__ R Ws[NW0 + NW1 + NW2];
     __shared__ Cx<R> X[CPB * XS];
     __shared__ Cx<R> Y[CPB * YS];
     __shared__ unsigned char subl[CPB][32];
     __shared__ int nsub[CPB];
     const int tid = threadIdx.x;
-    for (int i = tid; i < NW0 + NW1 + NW2; i += 256)
-        Ws[i] = W[i];
     const int64_t first = c0 + xcd_block() * CPB;
     const int ncone = int(min<int64_t>(CPB, c1 - first));
     for (int e = tid; e < ncone * NH; e += 256)
@@ -379,9 +376,11 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
         nsub[tid] = n;
     }
     __syncthreads();
-    const R* W0 = Ws;
-    const R* W1 = Ws + NW0;
-    const R* W2 = Ws + NW0 + NW1;
+    // the axis maps at compile-time offsets from the kernel argument: scalar loads, the
+    // weights stay in scalar registers as FMA operands
+    const R* W0 = W;
+    const R* W1 = W + NW0;
+    const R* W2 = W + NW0 + NW1;
     // axis 0: pencils (i1, i2): Y[j0 + M0 (i1 + H1 i2)] = sum_i0 W0[j0][i0] X[i0 + H0 (i1 + H1 i2)]
     for (int e = tid; e < ncone * H1 * H2; e += 256)
     {
