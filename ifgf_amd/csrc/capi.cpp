@@ -501,36 +501,38 @@ extern "C"
         return IFGF_OK;
     }
 
-    int ifgf_get_launch_shape(const ifgf_operator* op, int* shape4)
+    int ifgf_get_launch_shape(const ifgf_operator* op, int* shape)
     {
-        if (!op || !shape4)
+        if (!op || !shape)
             return IFGF_EINVAL;
         const LaunchShape& s = op->dev.shape;
-        shape4[0] = s.xcd_log2;
-        shape4[1] = s.cpb;
-        shape4[2] = s.up_k;
-        shape4[3] = s.rb;
+        shape[0] = s.xcd_log2;
+        shape[1] = s.cpb;
+        shape[2] = s.up_k;
+        shape[3] = s.rb;
+        shape[4] = s.far_waves;
         return IFGF_OK;
     }
 
-    int ifgf_set_launch_shape(ifgf_operator* op, const int* shape4)
+    int ifgf_set_launch_shape(ifgf_operator* op, const int* shape)
     {
-        if (!op || !shape4)
+        if (!op || !shape)
             return IFGF_EINVAL;
         if (!op->on_device)
             return fail(op, IFGF_ESTATE, "launch shapes need a device init");
         LaunchShape s;
-        s.xcd_log2 = shape4[0];
-        s.cpb = shape4[1];
-        s.up_k = shape4[2];
-        s.rb = shape4[3];
+        s.xcd_log2 = shape[0];
+        s.cpb = shape[1];
+        s.up_k = shape[2];
+        s.rb = shape[3];
+        s.far_waves = shape[4];
         std::string e = device_set_shape(op->dev, s);
         if (!e.empty())
             return fail(op, IFGF_EINVAL, e);
         return IFGF_OK;
     }
 
-    int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape4)
+    int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape)
     {
         if (!op)
             return IFGF_EINVAL;
@@ -542,6 +544,6 @@ extern "C"
         std::string e = device_autotune(op->dev, nrhs, reps, best);
         if (!e.empty())
             return fail(op, IFGF_EINTERNAL, e);
-        return shape4 ? ifgf_get_launch_shape(op, shape4) : IFGF_OK;
+        return shape ? ifgf_get_launch_shape(op, shape) : IFGF_OK;
     }
 }

@@ -565,8 +565,12 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
 // centre factor (ifgfoperator.hpp:962-1012, 1708-1891).  FAST (mixed form): cones located by
 // locate_fast with the level's fine-grid angle tables (doubles [ang_base, ang_base + ang_n) of
 // ang, shared by every source box of the level) staged in LDS; otherwise the exact locate.
-template <bool PH, typename R, int O0, int O1, int O2, bool FAST>
-__global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, int64_t nitems,
+template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int WPB>
+// the mixed order-6 kernel fits 64 VGPRs without spilling: ask for 8 waves per SIMD (the far
+// field is latency-bound; 74 registers would allow 6)
+__global__ __launch_bounds__(64 * WPB)
+__attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) void k_far(const WorkItem* __restrict__ items,
+                                                 int64_t nitems,
                                             const int32_t* __restrict__ far_off,
                                             const int32_t* __restrict__ far_src,
                                             const BoxGeo* __restrict__ geo,
@@ -577,16 +581,21 @@ __global__ __launch_bounds__(64) void k_far(const WorkItem* __restrict__ items, 
                                             int ang_n, Cx<R>* __restrict__ result, KParams kp)
 {
     extern __shared__ double sang[];
-    const int64_t it = xcd_block();
-    if (it >= nitems)
+    // WPB waves per workgroup, one item each, sharing the staged angle tables (one copy per
+    // workgroup: the tables, not the registers, would otherwise bound the waves per CU)
+    const int lane = WPB == 1 ? int(threadIdx.x) : int(threadIdx.x & 63);
+    const int64_t it =
+        WPB == 1 ? xcd_block() : xcd_block() * WPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (WPB == 1 && it >= nitems)
         return;
-    const int lane = threadIdx.x;
     if constexpr (FAST)
     {
-        for (int i = lane; i < ang_n; i += 64)
+        for (int i = threadIdx.x; i < ang_n; i += 64 * WPB)
             sang[i] = ang[ang_base + i];
         __syncthreads();
     }
+    if (WPB > 1 && it >= nitems)
+        return;
     const WorkItem wi = items[it];
     const int Tp = pow2_at_least(wi.count);
     const int ti = lane & (Tp - 1);
@@ -736,18 +745,34 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     else
     {
         const int abase = D.far_ang_base[far_level], an = D.far_ang_n[far_level];
+        const int64_t n = b - a;
+#define IFGF_FAR(FAST, WPB, LDS, BASE, AN)                                                      \
+    k_far<PH, R, O0, O1, O2, FAST, WPB>                                                         \
+        <<<dim3(unsigned((n + WPB - 1) / WPB)), dim3(64 * WPB), LDS, st>>>(                     \
+            D.far_items + a, n, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts, D.fine_ang,  \
+            BASE, AN, res, kp)
         if constexpr (sizeof(R) == 4)
             if (an > 0)
             {
-                k_far<PH, R, O0, O1, O2, true>
-                    <<<dim3(unsigned(b - a)), dim3(64), size_t(an) * sizeof(double), st>>>(
-                        D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F,
-                        D.tgt_pts, D.fine_ang, abase, an, res, kp);
+                const size_t lds = size_t(an) * sizeof(double);
+                switch (D.shape.far_waves)
+                {
+                case 1:
+                    IFGF_FAR(true, 1, lds, abase, an);
+                    break;
+                case 2:
+                    IFGF_FAR(true, 2, lds, abase, an);
+                    break;
+                case 8:
+                    IFGF_FAR(true, 8, lds, abase, an);
+                    break;
+                default:
+                    IFGF_FAR(true, 4, lds, abase, an);
+                }
                 return;
             }
-        k_far<PH, R, O0, O1, O2, false><<<dim3(unsigned(b - a)), dim3(64), 0, st>>>(
-            D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts,
-            D.fine_ang, 0, 0, res, kp);
+        IFGF_FAR(false, 1, 0, 0, 0);
+#undef IFGF_FAR
     }
 }
 
@@ -1546,6 +1571,8 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
         return "launch shape: upward cones per item must be 4, 8 or 16";
     if (sh.rb != 0 && sh.rb != 8 && sh.rb != 32)
         return "launch shape: batched columns per pass must be 8 or 32";
+    if (sh.far_waves != 1 && sh.far_waves != 2 && sh.far_waves != 4 && sh.far_waves != 8)
+        return "launch shape: far-field waves per workgroup must be 1, 2, 4 or 8";
     if (sh.cpb != 0)
     {
         const auto ch = transform_cpb_choices(order_class(D), D.prec == PREC_FP64 ? 8 : 4);
@@ -1554,7 +1581,7 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
                    "these orders and this precision";
     }
     const bool changed = sh.xcd_log2 != D.shape.xcd_log2 || sh.cpb != D.shape.cpb ||
-                         sh.up_k != D.shape.up_k;
+                         sh.up_k != D.shape.up_k || sh.far_waves != D.shape.far_waves;
     D.shape = sh;
     D.up_items = D.up_items_k[kc];
     D.up_item_off = D.up_item_off_k[kc];
@@ -1650,6 +1677,8 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
                 pick(&LaunchShape::cpb, ch, 3);
         }
         pick(&LaunchShape::up_k, {4, 8, 16}, batched ? -1 : 2);
+        if (!batched && D.prec == PREC_MIXED)
+            pick(&LaunchShape::far_waves, {1, 2, 4, 8}, 4);
         if (batched && nrhs > 8)
             pick(&LaunchShape::rb, {8, 32}, -1);
         err = device_set_shape(D, cur);

@@ -67,6 +67,7 @@ struct LaunchShape
     int cpb = 0;      // transform: cones per workgroup (0: the order class's default)
     int up_k = 8;     // upward: parent cones of one cell per workgroup (4, 8 or 16)
     int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
+    int far_waves = 4; // mixed far field: waves (items) per workgroup sharing the LDS tables
 };
 constexpr int UP_K_CHOICES[3] = {4, 8, 16};
 
@@ -177,8 +178,9 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
 std::string device_set_shape(DevicePlan& D, const LaunchShape& s);
 // Time the candidate shapes on this operator with synthetic densities (nrhs columns, each
 // candidate `reps` times, the fastest kept), one choice after another: the XCD run length on
-// the whole mult, then the transform's cones per workgroup and the upward cones per item on
-// their phases, then (nrhs >= 2) the batched columns per pass.  Leaves the best shape applied.
+// the whole mult, then the transform's cones per workgroup, the upward cones per item and the
+// far field's waves per workgroup on their phases, then (nrhs >= 2) the batched columns per
+// pass.  Leaves the best shape applied.
 std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best);
 void set_xcd_log2_batched(int lg);
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total

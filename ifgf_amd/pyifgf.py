@@ -211,19 +211,19 @@ class _IfgfOperator:
         check(lib().ifgf_last_timings(self._h, t.ctypes.data), self._h, "last_timings")
         return dict(zip(["near", "leaf", "upward", "transform", "far", "total"], t))
 
-    _SHAPE_KEYS = ("xcd_run_log2", "transform_cones", "up_cones", "batch_columns")
+    _SHAPE_KEYS = ("xcd_run_log2", "transform_cones", "up_cones", "batch_columns", "far_waves")
 
     def autotune(self, nrhs=1, reps=3):
         """Time the launch-shape candidates on this operator (synthetic densities, nrhs
         columns) and keep the fastest; returns the chosen shape."""
-        s = np.zeros(4, np.int32)
+        s = np.zeros(len(self._SHAPE_KEYS), np.int32)
         check(lib().ifgf_autotune(self._h, int(nrhs), int(reps), s.ctypes.data), self._h,
               "autotune")
         return dict(zip(self._SHAPE_KEYS, (int(v) for v in s)))
 
     @property
     def launch_shape(self):
-        s = np.zeros(4, np.int32)
+        s = np.zeros(len(self._SHAPE_KEYS), np.int32)
         check(lib().ifgf_get_launch_shape(self._h, s.ctypes.data), self._h, "launch_shape")
         return dict(zip(self._SHAPE_KEYS, (int(v) for v in s)))
 

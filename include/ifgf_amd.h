@@ -132,17 +132,20 @@ int ifgf_plan_work(const ifgf_operator* op, double* out10);
 int ifgf_last_timings(const ifgf_operator* op, double* ms6);
 
 /* --- launch shapes (no counterpart in the reference: its TBB loops have no tile shapes) ---
- * shape4: [0] XCD run log2 (work items dealt to the 8 XCDs in runs of 2^v; 0 = plain order)
+ * shape: IFGF_LAUNCH_SHAPE_FIELDS ints
+ *         [0] XCD run log2 (work items dealt to the 8 XCDs in runs of 2^v; 0 = plain order)
  *         [1] transform cones per workgroup (0 = default; orders 6 and 8 only)
  *         [2] upward parent cones per workgroup (4, 8, 16)
  *         [3] batched columns per pass (0 = by nrhs, 8, 32)
+ *         [4] mixed far field: waves per workgroup sharing the staged angle tables (1, 2, 4, 8)
  * Shapes change speed, never results.  ifgf_autotune times the candidates on the operator
  * itself (synthetic densities, nrhs columns, the fastest of reps mults per candidate), one
  * choice after another, applies the best and returns it; after ifgf_init, not thread-safe
  * with concurrent mults of the same operator. */
-int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape4);
-int ifgf_get_launch_shape(const ifgf_operator* op, int* shape4);
-int ifgf_set_launch_shape(ifgf_operator* op, const int* shape4);
+#define IFGF_LAUNCH_SHAPE_FIELDS 5
+int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape);
+int ifgf_get_launch_shape(const ifgf_operator* op, int* shape);
+int ifgf_set_launch_shape(ifgf_operator* op, const int* shape);
 
 #ifdef __cplusplus
 }

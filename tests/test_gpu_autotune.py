@@ -22,10 +22,12 @@ def test_every_shape_gives_the_same_result(gpu, prec, order):
     op = ifgf_amd.HelmholtzIfgfOperator(12.0, 64, order, 1, -1, precision=prec)
     op.init(x.T, x.T)
     ref = op.mult(w)
-    for lg, cpb, k in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16]):
-        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k)
-        print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k}", flush=True)
-        assert np.array_equal(op.mult(w), ref), (lg, cpb, k)
+    waves = [1, 2, 4, 8] if prec == "mixed" else [4]
+    for lg, cpb, k, fw in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16], waves):
+        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k, far_waves=fw)
+        print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k} far_waves={fw}",
+              flush=True)
+        assert np.array_equal(op.mult(w), ref), (lg, cpb, k, fw)
 
 
 def test_batched_shapes_and_autotune(gpu):
@@ -70,3 +72,5 @@ def test_autotune_single_column_and_graph_recapture(gpu):
         op.launch_shape = dict(up_cones=5)
     with pytest.raises(ifgf_amd.IfgfError):
         op.launch_shape = dict(transform_cones=7)
+    with pytest.raises(ifgf_amd.IfgfError):
+        op.launch_shape = dict(far_waves=3)
