@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no tuning after init)")
+    ap.add_argument("--launch-shape", type=str, default=None,
+                    help="JSON dict of launch shapes to apply instead of tuning, e.g. "
+                         "'{\"xcd_run_log2\": 4, \"up_cones\": 16}'")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device once (a plain copy kernel of "
                          "known size, for scaling profiler byte counters)")
@@ -161,7 +164,13 @@ def main():
     print(f"# init {t_init:.1f}s  {op.plan_counts()}", file=sys.stderr, flush=True)
     # launch shapes: timed on this operator once after init (untimed, like init)
     t0 = time.time()
-    shape = op.launch_shape if args.no_autotune else op.autotune(nrhs=1, reps=3)
+    if args.launch_shape:
+        op.launch_shape = json.loads(args.launch_shape)
+        shape = op.launch_shape
+    elif args.no_autotune:
+        shape = op.launch_shape
+    else:
+        shape = op.autotune(nrhs=1, reps=3)
     t_tune = time.time() - t0
     print(f"# launch shape {shape} (autotune {t_tune:.1f}s)", file=sys.stderr, flush=True)
     w_dev = torch.as_tensor(w, dtype=torch.complex128, device=dev)
@@ -197,7 +206,7 @@ def main():
         op64 = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
                                               precision="fp64")
         op64.init(x.T, x.T)
-        if not args.no_autotune:
+        if not (args.no_autotune or args.launch_shape):
             op64.autotune(nrhs=1, reps=2)
         ms64, u64 = time_mults(op64, w_dev, max(2, args.steps // 3), 1)
         u64_np = u64.cpu().numpy()

@@ -796,7 +796,7 @@ void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeT
 static std::vector<int> transform_cpb_choices(int cls, size_t rsize)
 {
     if (cls == 1)
-        return rsize == 4 ? std::vector<int>{6, 2, 4, 8} : std::vector<int>{3, 1, 2, 4};
+        return rsize == 4 ? std::vector<int>{4, 2, 6, 8} : std::vector<int>{3, 1, 2, 4};
     if (cls == 2)
         return rsize == 4 ? std::vector<int>{2, 1, 3} : std::vector<int>{1};
     return {0};
@@ -1360,8 +1360,8 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             D.up_item_off_k[kc][D.levels] = int64_t(uitems[kc].size());
             D.up_items_k[kc] = dupload(uitems[kc], tot);
         }
-        D.up_items = D.up_items_k[1];
-        D.up_item_off = D.up_item_off_k[1];
+        D.up_items = D.up_items_k[2]; // the default shape: 16 cones per item
+        D.up_item_off = D.up_item_off_k[2];
         D.up_glist = dupload(glist, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());

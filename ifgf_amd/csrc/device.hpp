@@ -65,7 +65,7 @@ struct LaunchShape
 {
     int xcd_log2 = 4; // work items dealt to the XCDs in runs of 2^xcd_log2 (0: plain order)
     int cpb = 0;      // transform: cones per workgroup (0: the order class's default)
-    int up_k = 8;     // upward: parent cones of one cell per workgroup (4, 8 or 16)
+    int up_k = 16;    // upward: parent cones of one cell per workgroup (4, 8 or 16)
     int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
     int far_waves = 4; // mixed far field: waves (items) per workgroup sharing the LDS tables
 };

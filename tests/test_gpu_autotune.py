@@ -11,7 +11,7 @@ from conftest import cdens, sphere, spheroid
 
 pytestmark = pytest.mark.gpu
 
-CPB = {("mixed", 6): [6, 2, 4, 8], ("fp64", 6): [3, 1, 2, 4], ("mixed", 8): [2, 1, 3],
+CPB = {("mixed", 6): [4, 2, 6, 8], ("fp64", 6): [3, 1, 2, 4], ("mixed", 8): [2, 1, 3],
        ("fp64", 8): [1]}
 
 
