@@ -333,6 +333,156 @@ __global__ __launch_bounds__(256) void k_up_cell_b(const WorkItem* __restrict__ 
     }
 }
 
+// upward pass with tables, order 6 (the batched form of k_up_cell with the children's
+// coefficients staged in LDS).  One workgroup per item (cell, up to K parent cones of that
+// cell); for each cone and each child in order: the nodes' fine cones in the child (table
+// entry + cell table) are collected, the distinct ones (up to UPS_CACHE) copied to LDS once as
+// N x RB coefficient blocks, and thread = (node set, column) evaluates every node from there
+// (a fine cone beyond the cache is read from memory: the same values).  Without the staging,
+// each (node, child, column) read its 18 coefficients from L2, which the 4.6 KB blocks of 32
+// columns thrash.
+constexpr int UPS_CACHE = 8;  // coefficient blocks staged per (cone, child)
+constexpr int UPS_HASH = 32; // open-addressing table of the distinct fine cones
+template <typename R, int O0, int O1, int O2, int NH, int K, int RB>
+__global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__ items,
+                                                    int64_t nitems,
+                                                    const int32_t* __restrict__ glist,
+                                                    const ConeDev* __restrict__ cones,
+                                                    const int32_t* __restrict__ child,
+                                                    const BoxGeo* __restrict__ geo,
+                                                    const CellRef* __restrict__ ctab,
+                                                    const UpEnt<R>* __restrict__ ents,
+                                                    const int64_t* __restrict__ part_ent,
+                                                    const Cx<R>* __restrict__ F,
+                                                    Cx<R>* __restrict__ V)
+{
+    constexpr int N = O0 * O1 * O2;
+    constexpr int TQ = 256 / RB;             // node lanes
+    constexpr int NQ = (NH + TQ - 1) / TQ;   // nodes per thread
+    static_assert(NH <= 256, "one thread per node while collecting");
+    __shared__ Cx<R> s_coef[UPS_CACHE * N * RB];
+    __shared__ int32_t s_hash[UPS_HASH];     // fine cone (slot) or -1
+    __shared__ int32_t s_cidx[UPS_HASH];     // its cache index or -1
+    __shared__ int32_t s_cached[UPS_CACHE];  // slots in the cache
+    __shared__ int32_t s_ncache;
+    __shared__ int32_t s_qslot[NH];          // fine cone of each node in this child (or -1)
+    __shared__ int32_t s_qh[NH];             // its hash position
+    __shared__ R s_qx[3][NH];
+    __shared__ Cx<R> s_qtf[NH];
+    const int64_t it = xcd_block();
+    if (it >= nitems)
+        return;
+    const WorkItem wi = items[it];
+    const int tid = threadIdx.x;
+    const int r = tid % RB, tq = tid / RB;
+    const UpEnt<R>* ebase = ents + part_ent[wi.pad] + int64_t(wi.box) * NH * 8;
+    for (int k = 0; k < wi.count; k++)
+    {
+        const int32_t g = glist[wi.first + k];
+        const int32_t pbox = cones[g].box;
+        Cx<R> acc[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; j++)
+            acc[j] = {0, 0};
+#pragma unroll 1
+        for (int c = 0; c < 8; c++)
+        {
+            const int32_t ch = child[8 * pbox + c];
+            if (ch < 0)
+                continue; // uniform: the same child for every thread
+            if (tid < UPS_HASH)
+                s_hash[tid] = -1;
+            if (tid == 0)
+                s_ncache = 0;
+            __syncthreads();
+            // collect: one thread per node
+            if (tid < NH)
+            {
+                const UpEnt<R> E = ebase[int64_t(tid) * 8 + c];
+                int slot = -1;
+                if (E.cf >= 0)
+                {
+                    const CellRef cr = ctab[geo[ch].table_off + E.cf];
+                    if ((cr.mask >> E.sub) & 1u)
+                        slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
+                }
+                int h = -1; // hash position of the slot; -1: not in the table (read from memory)
+                if (slot >= 0)
+                {
+                    int at = slot & (UPS_HASH - 1);
+                    for (int probe = 0; probe < UPS_HASH; probe++)
+                    {
+                        const int32_t prev = atomicCAS(&s_hash[at], -1, slot);
+                        if (prev == -1 || prev == slot)
+                        {
+                            h = at;
+                            break;
+                        }
+                        at = (at + 1) & (UPS_HASH - 1);
+                    }
+                }
+                s_qslot[tid] = slot;
+                s_qh[tid] = h;
+                s_qx[0][tid] = E.xi0;
+                s_qx[1][tid] = E.xi1;
+                s_qx[2][tid] = E.xi2;
+                s_qtf[tid] = Cx<R>{E.tr, E.ti};
+            }
+            __syncthreads();
+            // cache indices of the distinct slots (table order)
+            if (tid < UPS_HASH)
+            {
+                const int32_t sl = s_hash[tid];
+                int ci = -1;
+                if (sl >= 0)
+                {
+                    ci = atomicAdd(&s_ncache, 1);
+                    if (ci < UPS_CACHE)
+                        s_cached[ci] = sl;
+                    else
+                        ci = -1;
+                }
+                s_cidx[tid] = ci;
+            }
+            __syncthreads();
+            const int nc = min(s_ncache, UPS_CACHE);
+            for (int e = tid; e < nc * N * RB; e += 256)
+            {
+                const int b = e / (N * RB), o = e - b * (N * RB);
+                s_coef[e] = F[int64_t(s_cached[b]) * N * RB + o];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NQ; j++)
+            {
+                const int q = tq + j * TQ;
+                if (q >= NH)
+                    break;
+                const int slot = s_qslot[q];
+                if (slot < 0)
+                    continue;
+                const int h = s_qh[q];
+                const int ci = h >= 0 ? s_cidx[h] : -1;
+                const Cx<R>* blk = ci >= 0 ? s_coef + ci * N * RB : F + int64_t(slot) * N * RB;
+                Cx<R> cf[N];
+#pragma unroll
+                for (int i = 0; i < N; i++)
+                    cf[i] = blk[i * RB + r];
+                const Cx<R> val = clen3c<O0, O1, O2, R>(cf, s_qx[0][q], s_qx[1][q], s_qx[2][q]);
+                acc[j] = cadd(acc[j], cmul(val, s_qtf[q]));
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < NQ; j++)
+        {
+            const int q = tq + j * TQ;
+            if (q < NH)
+                V[(int64_t(g) * NH + q) * RB + r] = acc[j];
+        }
+    }
+}
+
 // upward pass without tables (k_up): thread = (node, column); each column lane locates the
 // node in the children's grids itself
 template <bool PH, typename R, int O0, int O1, int O2, int RB>
@@ -698,7 +848,29 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
         const int64_t i0 = D.up_item_off[l], i1 = D.up_item_off[l + 1];
-        if (i1 > i0)
+        if constexpr (P6 && sizeof(R) == 4)
+        {
+            if (i1 > i0)
+            {
+#define IFGF_UP_STAGE_B(K)                                                                      \
+    k_up_stage_b<R, O0, O1, O2, 108, K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(    \
+        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[l], D.cones, D.child, D.geo, D.ctab, \
+        static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V)
+            switch (D.shape.up_k)
+            {
+            case 4:
+                IFGF_UP_STAGE_B(4);
+                break;
+            case 16:
+                IFGF_UP_STAGE_B(16);
+                break;
+            default:
+                IFGF_UP_STAGE_B(8);
+            }
+#undef IFGF_UP_STAGE_B
+            }
+        }
+        else if (i1 > i0)
         {
 #define IFGF_UP_CELL_B(K)                                                                       \
     k_up_cell_b<R, O0, O1, O2, K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(           \
