@@ -511,6 +511,7 @@ extern "C"
         shape[2] = s.up_k;
         shape[3] = s.rb;
         shape[4] = s.far_waves;
+        shape[5] = s.overlap;
         return IFGF_OK;
     }
 
@@ -526,6 +527,7 @@ extern "C"
         s.up_k = shape[2];
         s.rb = shape[3];
         s.far_waves = shape[4];
+        s.overlap = shape[5];
         std::string e = device_set_shape(op->dev, s);
         if (!e.empty())
             return fail(op, IFGF_EINVAL, e);

@@ -950,12 +950,23 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
+    // the near field only writes the result (after the zeroing) and the far field adds to it
+    // (after the near field): with `overlap` it runs on the side stream, forked after the leaf
+    // sampling and the deepest level's transform (VALU- and LDS-bound like it) so that it runs
+    // beside the upward passes (its waves fill the memory-bound upward kernels); phase timing
+    // runs everything in order on one stream
+    const bool overlap = D.shape.overlap && !pt.on && D.n_near_items && D.side_stream;
+    hipStream_t side = static_cast<hipStream_t>(D.side_stream);
     pt.begin(0);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
-    if (D.n_near_items)
-        k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
+    auto near = [&](hipStream_t s)
+    {
+        k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, s>>>(
             D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin, D.src_end,
             D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
+    };
+    if (D.n_near_items && !overlap)
+        near(st);
     pt.end();
     pt.begin(1);
     if (D.n_leaf_cones)
@@ -963,6 +974,17 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
             D.normals, ws, tabs, V, kp);
     pt.end();
+    bool forked = false;
+    auto fork_near = [&]()
+    {
+        if (!overlap || forked)
+            return;
+        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_fork), st));
+        HIP_OK(hipStreamWaitEvent(side, static_cast<hipEvent_t>(D.ev_fork), 0));
+        near(side);
+        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_near), side));
+        forked = true;
+    };
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
@@ -978,8 +1000,12 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             pt.begin(3);
             launch_transform<R>(D, st, kp, c0, c1, V, F);
             pt.end();
+            fork_near();
         }
     }
+    fork_near(); // a plan without cones
+    if (overlap)
+        HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
     pt.begin(4);
     for (size_t l = 0; l + 1 < D.far_item_off.size(); l++)
     {
@@ -998,6 +1024,18 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     PhaseTimer pt{timed, st, {}};
+    if (D.shape.overlap && !D.side_stream)
+    {
+        // created outside any capture; default priority (as every stream here)
+        hipStream_t s2;
+        hipEvent_t e1, e2;
+        HIP_OK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+        D.side_stream = s2;
+        D.ev_fork = e1;
+        D.ev_near = e2;
+    }
     const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
     const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
     const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
@@ -1086,6 +1124,12 @@ void device_free(DevicePlan& D)
         (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
     if (D.cap_stream)
         (void)hipStreamDestroy(static_cast<hipStream_t>(D.cap_stream));
+    if (D.side_stream)
+        (void)hipStreamDestroy(static_cast<hipStream_t>(D.side_stream));
+    if (D.ev_fork)
+        (void)hipEventDestroy(static_cast<hipEvent_t>(D.ev_fork));
+    if (D.ev_near)
+        (void)hipEventDestroy(static_cast<hipEvent_t>(D.ev_near));
     D = DevicePlan();
 }
 
@@ -1573,6 +1617,8 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
         return "launch shape: batched columns per pass must be 8 or 32";
     if (sh.far_waves != 1 && sh.far_waves != 2 && sh.far_waves != 4 && sh.far_waves != 8)
         return "launch shape: far-field waves per workgroup must be 1, 2, 4 or 8";
+    if (sh.overlap != 0 && sh.overlap != 1)
+        return "launch shape: overlap is 0 or 1";
     if (sh.cpb != 0)
     {
         const auto ch = transform_cpb_choices(order_class(D), D.prec == PREC_FP64 ? 8 : 4);
@@ -1581,7 +1627,8 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
                    "these orders and this precision";
     }
     const bool changed = sh.xcd_log2 != D.shape.xcd_log2 || sh.cpb != D.shape.cpb ||
-                         sh.up_k != D.shape.up_k || sh.far_waves != D.shape.far_waves;
+                         sh.up_k != D.shape.up_k || sh.far_waves != D.shape.far_waves ||
+                         sh.overlap != D.shape.overlap;
     D.shape = sh;
     D.up_items = D.up_items_k[kc];
     D.up_item_off = D.up_item_off_k[kc];
@@ -1679,6 +1726,8 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
         pick(&LaunchShape::up_k, {4, 8, 16}, batched ? -1 : 2);
         if (!batched && D.prec == PREC_MIXED)
             pick(&LaunchShape::far_waves, {1, 2, 4, 8}, 4);
+        if (!batched)
+            pick(&LaunchShape::overlap, {0, 1}, -1);
         if (batched && nrhs > 8)
             pick(&LaunchShape::rb, {8, 32}, -1);
         err = device_set_shape(D, cur);

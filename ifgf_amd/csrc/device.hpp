@@ -68,6 +68,8 @@ struct LaunchShape
     int up_k = 16;    // upward: parent cones of one cell per workgroup (4, 8 or 16)
     int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
     int far_waves = 4; // mixed far field: waves (items) per workgroup sharing the LDS tables
+    int overlap = 1;   // single-column mult: near field on a second stream beside the upward
+                       // pass (VALU-bound beside memory-bound work), joined before the far field
 };
 constexpr int UP_K_CHOICES[3] = {4, 8, 16};
 
@@ -147,6 +149,10 @@ struct DevicePlan
     void* graph = nullptr;      // hipGraph_t
     void* graph_exec = nullptr; // hipGraphExec_t
     void* cap_stream = nullptr; // hipStream_t of the capture
+    // the second stream of the overlapped near field and its fork / join events
+    void* side_stream = nullptr;
+    void* ev_fork = nullptr;
+    void* ev_near = nullptr;
 
     LaunchShape shape;
 

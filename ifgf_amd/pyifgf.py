@@ -211,7 +211,8 @@ class _IfgfOperator:
         check(lib().ifgf_last_timings(self._h, t.ctypes.data), self._h, "last_timings")
         return dict(zip(["near", "leaf", "upward", "transform", "far", "total"], t))
 
-    _SHAPE_KEYS = ("xcd_run_log2", "transform_cones", "up_cones", "batch_columns", "far_waves")
+    _SHAPE_KEYS = ("xcd_run_log2", "transform_cones", "up_cones", "batch_columns", "far_waves",
+                   "overlap")
 
     def autotune(self, nrhs=1, reps=3):
         """Time the launch-shape candidates on this operator (synthetic densities, nrhs

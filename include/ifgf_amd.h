@@ -138,11 +138,12 @@ int ifgf_last_timings(const ifgf_operator* op, double* ms6);
  *         [2] upward parent cones per workgroup (4, 8, 16; default 16)
  *         [3] batched columns per pass (0 = by nrhs, 8, 32)
  *         [4] mixed far field: waves per workgroup sharing the staged angle tables (1, 2, 4, 8)
+ *         [5] one column: near field on a second stream beside the upward pass (0, 1)
  * Shapes change speed, never results.  ifgf_autotune times the candidates on the operator
  * itself (synthetic densities, nrhs columns, the fastest of reps mults per candidate), one
  * choice after another, applies the best and returns it; after ifgf_init, not thread-safe
  * with concurrent mults of the same operator. */
-#define IFGF_LAUNCH_SHAPE_FIELDS 5
+#define IFGF_LAUNCH_SHAPE_FIELDS 6
 int ifgf_autotune(ifgf_operator* op, int64_t nrhs, int reps, int* shape);
 int ifgf_get_launch_shape(const ifgf_operator* op, int* shape);
 int ifgf_set_launch_shape(ifgf_operator* op, const int* shape);

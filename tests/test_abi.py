@@ -39,11 +39,11 @@ def test_version_and_errors_without_device():
     assert L.ifgf_mult(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
     assert L.ifgf_mult_adjoint(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
     assert L.ifgf_set_dx(h, 1) == _lib.IFGF_EINVAL  # not a gradient operator
-    shape = np.zeros(5, np.int32)
+    shape = np.zeros(6, np.int32)
     assert L.ifgf_autotune(h, 1, 1, shape.ctypes.data) == _lib.IFGF_ESTATE
     assert L.ifgf_set_launch_shape(h, shape.ctypes.data) == _lib.IFGF_ESTATE
     assert L.ifgf_get_launch_shape(h, shape.ctypes.data) == _lib.IFGF_OK
-    assert shape.tolist() == [4, 0, 16, 0, 4]  # the defaults
+    assert shape.tolist() == [4, 0, 16, 0, 4, 1]  # the defaults
     if L.ifgf_device_count() == 0:
         x = np.random.default_rng(0).standard_normal((50, 3))
         rc = L.ifgf_init(h, x.ctypes.data, 50, x.ctypes.data, 50, None, _lib.CUTOFF_FN(0), None, 0)

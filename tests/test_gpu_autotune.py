@@ -23,11 +23,13 @@ def test_every_shape_gives_the_same_result(gpu, prec, order):
     op.init(x.T, x.T)
     ref = op.mult(w)
     waves = [1, 2, 4, 8] if prec == "mixed" else [4]
-    for lg, cpb, k, fw in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16], waves):
-        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k, far_waves=fw)
-        print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k} far_waves={fw}",
-              flush=True)
-        assert np.array_equal(op.mult(w), ref), (lg, cpb, k, fw)
+    for lg, cpb, k, fw, ov in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16], waves,
+                                                [0, 1]):
+        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k, far_waves=fw,
+                               overlap=ov)
+        print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k} far_waves={fw} "
+              f"overlap={ov}", flush=True)
+        assert np.array_equal(op.mult(w), ref), (lg, cpb, k, fw, ov)
 
 
 def test_batched_shapes_and_autotune(gpu):
@@ -74,3 +76,8 @@ def test_autotune_single_column_and_graph_recapture(gpu):
         op.launch_shape = dict(transform_cones=7)
     with pytest.raises(ifgf_amd.IfgfError):
         op.launch_shape = dict(far_waves=3)
+    # the overlapped near field inside a captured graph: replays against the eager mult
+    op.launch_shape = dict(overlap=1)
+    for _ in range(3):
+        g3 = op.mult(wd).cpu().numpy()
+        assert np.array_equal(g3, ref)
