@@ -950,11 +950,15 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
-    // the near field only writes the result (after the zeroing) and the far field adds to it
-    // (after the near field): with `overlap` it runs on the side stream, forked after the leaf
-    // sampling and the deepest level's transform (VALU- and LDS-bound like it) so that it runs
-    // beside the upward passes (its waves fill the memory-bound upward kernels); phase timing
-    // runs everything in order on one stream
+    // The near field only writes the result (after the zeroing) and the far field adds to it
+    // (after the near field), one target level after another, deepest first; the far field of
+    // a target level reads the coefficients of its source levels only.  With `overlap` the
+    // near field and the far field run on the side stream: the near field forked after the
+    // leaf sampling and the deepest level's transform (VALU- and LDS-bound like it), each far
+    // level as soon as the upward chain has written its shallowest source level, so the
+    // VALU-bound near and far waves run beside the memory-bound upward kernels.  The order of
+    // the additions is the same either way (results are identical).  Phase timing runs
+    // everything in order on one stream.
     const bool overlap = D.shape.overlap && !pt.on && D.n_near_items && D.side_stream;
     hipStream_t side = static_cast<hipStream_t>(D.side_stream);
     pt.begin(0);
@@ -982,8 +986,34 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
         HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_fork), st));
         HIP_OK(hipStreamWaitEvent(side, static_cast<hipEvent_t>(D.ev_fork), 0));
         near(side);
-        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_near), side));
         forked = true;
+    };
+    // far levels, deepest target level first; next: the first not yet enqueued
+    const int nfl = int(D.far_item_off.size()) - 1;
+    int next = nfl - 1;
+    auto far_level = [&](int l, hipStream_t s)
+    {
+        const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
+        if (b > a)
+            launch_up_far<PH, R>(D, s, kp, tabs, -1, l, a, b, F, V, res);
+    };
+    // after the upward chain has written source level `done` (all deeper ones before it)
+    auto far_ready = [&](int done)
+    {
+        if (!overlap || !forked)
+            return;
+        bool waited = false;
+        while (next >= 0 && D.far_min_src[next] >= done)
+        {
+            if (!waited)
+            {
+                hipEvent_t e = static_cast<hipEvent_t>(D.ev_level[done]);
+                HIP_OK(hipEventRecord(e, st));
+                HIP_OK(hipStreamWaitEvent(side, e, 0));
+                waited = true;
+            }
+            far_level(next--, side);
+        }
     };
     for (int l = D.levels - 1; l >= 0; l--)
     {
@@ -1002,17 +1032,20 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             pt.end();
             fork_near();
         }
+        far_ready(l);
     }
     fork_near(); // a plan without cones
+    far_ready(0);
     if (overlap)
-        HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
-    pt.begin(4);
-    for (size_t l = 0; l + 1 < D.far_item_off.size(); l++)
     {
-        const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
-        if (b > a)
-            launch_up_far<PH, R>(D, st, kp, tabs, -1, int(l), a, b, F, V, res);
+        // the side stream holds every far level now (level 0 is the last one written)
+        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_near), side));
+        HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
+        return;
     }
+    pt.begin(4);
+    for (int l = nfl - 1; l >= 0; l--)
+        far_level(l, st);
     pt.end();
 }
 
@@ -1035,6 +1068,13 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
         D.side_stream = s2;
         D.ev_fork = e1;
         D.ev_near = e2;
+        D.ev_level.assign(std::max(1, D.levels), nullptr);
+        for (void*& e : D.ev_level)
+        {
+            hipEvent_t ev;
+            HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            e = ev;
+        }
     }
     const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
     const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
@@ -1130,6 +1170,9 @@ void device_free(DevicePlan& D)
         (void)hipEventDestroy(static_cast<hipEvent_t>(D.ev_fork));
     if (D.ev_near)
         (void)hipEventDestroy(static_cast<hipEvent_t>(D.ev_near));
+    for (void* e : D.ev_level)
+        if (e)
+            (void)hipEventDestroy(static_cast<hipEvent_t>(e));
     D = DevicePlan();
 }
 
@@ -1441,6 +1484,7 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         D.far_item_off.assign(1, 0);
         D.far_ang_base.clear();
         D.far_ang_n.clear();
+        D.far_min_src.clear();
         for (int l = 0; l < P.T.levels; l++)
         {
             std::vector<int32_t> bl;
@@ -1465,6 +1509,11 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             const bool fits = hi > lo && (hi - lo) * int64_t(sizeof(double)) <= 48 * 1024;
             D.far_ang_base.push_back(fits ? int(lo) : 0);
             D.far_ang_n.push_back(fits ? int(hi - lo) : 0);
+            int min_src = P.S.levels - 1;
+            for (int32_t t : bl)
+                for (int32_t j = foff[t]; j < foff[t + 1]; j++)
+                    min_src = std::min(min_src, P.S.level_of(fsrc[j]));
+            D.far_min_src.push_back(min_src);
         }
         D.far_items = dupload(fitems, tot);
         auto nitems = make_items(ntb, P.T);

@@ -133,6 +133,7 @@ struct DevicePlan
     WorkItem* far_items = nullptr;
     std::vector<int64_t> far_item_off; // per target level
     std::vector<int> far_ang_base, far_ang_n; // per target level: angle tables staged in LDS
+    std::vector<int> far_min_src; // per target level: the shallowest source level it reads
     WorkItem* near_items = nullptr;
     int64_t n_near_items = 0;
 
@@ -153,6 +154,7 @@ struct DevicePlan
     void* side_stream = nullptr;
     void* ev_fork = nullptr;
     void* ev_near = nullptr;
+    std::vector<void*> ev_level; // per source level: its coefficients are written
 
     LaunchShape shape;
 
