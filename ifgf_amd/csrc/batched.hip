@@ -835,10 +835,25 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
     k_gather_b<R><<<dim3(unsigned((ns + 255) / 256)), dim3(256), 0, st>>>(
         w_dev, D.src_perm, ws, D.nsrc, RB, ncol, in_im);
     k_zero_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(res, nt);
-    if (D.n_near_items)
-        k_near_b<KIND, R, RB><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, st>>>(
-            D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin, D.src_end,
-            D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
+    // near and far fields as in the single-column mult (SidePipe)
+    const bool overlap = D.shape.overlap && D.n_near_items;
+    auto pipe = side_pipe(
+        D, st, overlap,
+        [&](hipStream_t s)
+        {
+            if (D.n_near_items)
+                k_near_b<KIND, R, RB><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, s>>>(
+                    D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin,
+                    D.src_end, D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
+        },
+        [&](int l, hipStream_t s)
+        {
+            const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
+            if (b > a)
+                launch_far_b<PH, R, O0, O1, O2, RB>(D, s, kp, l, a, b, F, res);
+        });
+    if (!overlap)
+        pipe.fork();
     if (D.n_leaf_cones)
         k_leaf_b<KIND, R, RB><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(
             D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
@@ -898,14 +913,11 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
                 launch_transform_b<R, 3, 6, 6, 2, 3, 3, NI, RB>(D, st, c0, c1, V, F);
             else
                 launch_transform_b<R, 5, 8, 8, 2, 5, 5, NI, RB>(D, st, c0, c1, V, F);
+            pipe.fork();
         }
+        pipe.ready(l);
     }
-    for (size_t l = 0; l + 1 < D.far_item_off.size(); l++)
-    {
-        const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
-        if (b > a)
-            launch_far_b<PH, R, O0, O1, O2, RB>(D, st, kp, int(l), a, b, F, res);
-    }
+    pipe.finish();
     k_scatter_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(
         res, D.tgt_perm, out_dev, D.ntgt, RB, ncol, out_re, out_im);
     HIP_OK(hipGetLastError());

@@ -729,6 +729,82 @@ static inline void set_xcd_log2(int lg)
     h_xcd_log2 = lg;
 }
 
+// The near field and the far levels of a mult on the plan's side stream (host side).  The
+// near field only writes the result (after the zeroing) and the far field adds to it (after
+// the near field), one target level after another, deepest first; the far field of a target
+// level reads the coefficients of its source levels only.  With `on`, fork() starts the near
+// field on the side stream (called after the deepest level's transform, so that it runs beside
+// the memory-bound upward kernels rather than the VALU-bound leaf sampling), ready(l), called
+// once the upward chain has written source level l, enqueues there every far level whose
+// shallowest source level is l or deeper, and finish() joins the side stream back.  Without
+// `on` the near field runs at fork() and the far levels at finish(), in the same order, on the
+// main stream (the sums are the same).
+template <class NearFn, class FarFn> struct SidePipe
+{
+    DevicePlan& D;
+    hipStream_t st;
+    bool on;
+    NearFn near; // near(stream)
+    FarFn far;   // far(target level, stream)
+    bool forked = false;
+    int next;    // the next far level to enqueue (deepest first)
+    SidePipe(DevicePlan& d, hipStream_t s, bool overlap, NearFn n, FarFn f)
+        : D(d), st(s), on(overlap && d.side_stream), near(n), far(f),
+          next(int(d.far_item_off.size()) - 2)
+    {
+    }
+    hipStream_t side() const { return static_cast<hipStream_t>(D.side_stream); }
+    void fork()
+    {
+        if (forked)
+            return;
+        forked = true;
+        if (!on)
+        {
+            near(st);
+            return;
+        }
+        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_fork), st));
+        HIP_OK(hipStreamWaitEvent(side(), static_cast<hipEvent_t>(D.ev_fork), 0));
+        near(side());
+    }
+    void ready(int done)
+    {
+        if (!on || !forked)
+            return;
+        bool waited = false;
+        while (next >= 0 && D.far_min_src[next] >= done)
+        {
+            if (!waited)
+            {
+                hipEvent_t e = static_cast<hipEvent_t>(D.ev_level[done]);
+                HIP_OK(hipEventRecord(e, st));
+                HIP_OK(hipStreamWaitEvent(side(), e, 0));
+                waited = true;
+            }
+            far(next--, side());
+        }
+    }
+    void finish()
+    {
+        fork();
+        if (on)
+        {
+            ready(0);
+            HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_near), side()));
+            HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
+            return;
+        }
+        for (; next >= 0; next--)
+            far(next, st);
+    }
+};
+template <class NearFn, class FarFn>
+SidePipe<NearFn, FarFn> side_pipe(DevicePlan& D, hipStream_t st, bool overlap, NearFn n, FarFn f)
+{
+    return SidePipe<NearFn, FarFn>(D, st, overlap, n, f);
+}
+
 __device__ __forceinline__ int pow2_at_least(int n)
 {
     int p = 1;

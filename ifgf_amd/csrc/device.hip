@@ -950,27 +950,28 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
-    // The near field only writes the result (after the zeroing) and the far field adds to it
-    // (after the near field), one target level after another, deepest first; the far field of
-    // a target level reads the coefficients of its source levels only.  With `overlap` the
-    // near field and the far field run on the side stream: the near field forked after the
-    // leaf sampling and the deepest level's transform (VALU- and LDS-bound like it), each far
-    // level as soon as the upward chain has written its shallowest source level, so the
-    // VALU-bound near and far waves run beside the memory-bound upward kernels.  The order of
-    // the additions is the same either way (results are identical).  Phase timing runs
-    // everything in order on one stream.
-    const bool overlap = D.shape.overlap && !pt.on && D.n_near_items && D.side_stream;
-    hipStream_t side = static_cast<hipStream_t>(D.side_stream);
+    // near and far fields: SidePipe (overlapped with the upward chain on the side stream when
+    // the shape asks for it; phase timing runs everything in order on one stream)
+    const bool overlap = D.shape.overlap && !pt.on && D.n_near_items;
     pt.begin(0);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
-    auto near = [&](hipStream_t s)
-    {
-        k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, s>>>(
-            D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin, D.src_end,
-            D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
-    };
-    if (D.n_near_items && !overlap)
-        near(st);
+    auto pipe = side_pipe(
+        D, st, overlap,
+        [&](hipStream_t s)
+        {
+            if (D.n_near_items)
+                k_near<KIND, R><<<dim3(unsigned(D.n_near_items)), dim3(64), 0, s>>>(
+                    D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin,
+                    D.src_end, D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
+        },
+        [&](int l, hipStream_t s)
+        {
+            const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
+            if (b > a)
+                launch_up_far<PH, R>(D, s, kp, tabs, -1, l, a, b, F, V, res);
+        });
+    if (!overlap)
+        pipe.fork(); // the near field first, in order (and inside the timed near phase)
     pt.end();
     pt.begin(1);
     if (D.n_leaf_cones)
@@ -978,43 +979,6 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
             D.normals, ws, tabs, V, kp);
     pt.end();
-    bool forked = false;
-    auto fork_near = [&]()
-    {
-        if (!overlap || forked)
-            return;
-        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_fork), st));
-        HIP_OK(hipStreamWaitEvent(side, static_cast<hipEvent_t>(D.ev_fork), 0));
-        near(side);
-        forked = true;
-    };
-    // far levels, deepest target level first; next: the first not yet enqueued
-    const int nfl = int(D.far_item_off.size()) - 1;
-    int next = nfl - 1;
-    auto far_level = [&](int l, hipStream_t s)
-    {
-        const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
-        if (b > a)
-            launch_up_far<PH, R>(D, s, kp, tabs, -1, l, a, b, F, V, res);
-    };
-    // after the upward chain has written source level `done` (all deeper ones before it)
-    auto far_ready = [&](int done)
-    {
-        if (!overlap || !forked)
-            return;
-        bool waited = false;
-        while (next >= 0 && D.far_min_src[next] >= done)
-        {
-            if (!waited)
-            {
-                hipEvent_t e = static_cast<hipEvent_t>(D.ev_level[done]);
-                HIP_OK(hipEventRecord(e, st));
-                HIP_OK(hipStreamWaitEvent(side, e, 0));
-                waited = true;
-            }
-            far_level(next--, side);
-        }
-    };
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
@@ -1030,22 +994,12 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             pt.begin(3);
             launch_transform<R>(D, st, kp, c0, c1, V, F);
             pt.end();
-            fork_near();
+            pipe.fork();
         }
-        far_ready(l);
-    }
-    fork_near(); // a plan without cones
-    far_ready(0);
-    if (overlap)
-    {
-        // the side stream holds every far level now (level 0 is the last one written)
-        HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_near), side));
-        HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
-        return;
+        pipe.ready(l);
     }
     pt.begin(4);
-    for (int l = nfl - 1; l >= 0; l--)
-        far_level(l, st);
+    pipe.finish();
     pt.end();
 }
 
@@ -1057,25 +1011,6 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
     Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
     Cx<R>* res = static_cast<Cx<R>*>(D.result);
     PhaseTimer pt{timed, st, {}};
-    if (D.shape.overlap && !D.side_stream)
-    {
-        // created outside any capture; default priority (as every stream here)
-        hipStream_t s2;
-        hipEvent_t e1, e2;
-        HIP_OK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
-        D.side_stream = s2;
-        D.ev_fork = e1;
-        D.ev_near = e2;
-        D.ev_level.assign(std::max(1, D.levels), nullptr);
-        for (void*& e : D.ev_level)
-        {
-            hipEvent_t ev;
-            HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            e = ev;
-        }
-    }
     const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
     const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
     const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
@@ -1581,6 +1516,26 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
         const bool timed = g_force_timing || std::getenv("IFGF_PHASE_TIMING") != nullptr;
         dev::set_xcd_log2(D.shape.xcd_log2);
         set_xcd_log2_batched(D.shape.xcd_log2);
+        if (D.shape.overlap && !D.side_stream)
+        {
+            // the side stream of the overlapped near and far fields and its events, created
+            // outside any capture; default priority (as every stream here)
+            hipStream_t s2;
+            hipEvent_t e1, e2;
+            HIP_OK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            HIP_OK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+            D.side_stream = s2;
+            D.ev_fork = e1;
+            D.ev_near = e2;
+            D.ev_level.assign(std::max(1, D.levels), nullptr);
+            for (void*& e : D.ev_level)
+            {
+                hipEvent_t ev;
+                HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                e = ev;
+            }
+        }
         // several columns: the batched pipeline (8 or 32 columns per pass) when the orders
         // have compiled kernels and the batch buffers fit
         if (nrhs >= 2 && batched_supported(D) && std::getenv("IFGF_NO_BATCH") == nullptr)
