@@ -658,7 +658,7 @@ __global__ __launch_bounds__(256) void k_transform_b(int64_t c0, int64_t c1,
 template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int RB>
 __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ items,
                                                int64_t nitems,
-                                               const int32_t* __restrict__ far_off,
+                                               const int32_t* __restrict__ srange,
                                                const int32_t* __restrict__ far_src,
                                                const BoxGeo* __restrict__ geo,
                                                const CellRef* __restrict__ ctab,
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
 #pragma unroll
     for (int j = 0; j < TPT; j++)
         acc[j] = {0, 0};
-    const int32_t j0 = far_off[wi.box], j1 = far_off[wi.box + 1];
+    const int32_t j0 = srange[2 * it], j1 = srange[2 * it + 1];
     for (int32_t jb = j0; jb < j1; jb += 4)
     {
         {
@@ -790,19 +790,20 @@ template <bool PH, typename R, int O0, int O1, int O2, int RB>
 void launch_far_b(DevicePlan& D, hipStream_t st, const KParams& kp, int l, int64_t a, int64_t b,
                   const Cx<R>* F, Cx<R>* res)
 {
-    const int abase = D.far_ang_base[l], an = D.far_ang_n[l];
+    // l: the far group; a, b: its items
+    const int abase = D.far_groups[l].ang_base, an = D.far_groups[l].ang_n;
     if constexpr (sizeof(R) == 4)
         if (an > 0)
         {
             k_far_b<PH, R, O0, O1, O2, true, RB>
                 <<<dim3(unsigned(b - a)), dim3(256), size_t(an) * sizeof(double), st>>>(
-                    D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts,
-                    D.fine_ang, abase, an, res, kp);
+                    D.far_items + a, b - a, D.far_srange + 2 * a, D.far_src, D.geo, D.ctab, F,
+                    D.tgt_pts, D.fine_ang, abase, an, res, kp);
             return;
         }
     k_far_b<PH, R, O0, O1, O2, false, RB><<<dim3(unsigned(b - a)), dim3(256), 0, st>>>(
-        D.far_items + a, b - a, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts, D.fine_ang,
-        0, 0, res, kp);
+        D.far_items + a, b - a, D.far_srange + 2 * a, D.far_src, D.geo, D.ctab, F, D.tgt_pts,
+        D.fine_ang, 0, 0, res, kp);
 }
 
 template <typename R, int H0, int H1, int H2, int L0, int L1, int L2, int NI, int RB>
@@ -846,11 +847,10 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
                     D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin,
                     D.src_end, D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
         },
-        [&](int l, hipStream_t s)
+        [&](int g, hipStream_t s)
         {
-            const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
-            if (b > a)
-                launch_far_b<PH, R, O0, O1, O2, RB>(D, s, kp, l, a, b, F, res);
+            const DevicePlan::FarGroup& G = D.far_groups[g];
+            launch_far_b<PH, R, O0, O1, O2, RB>(D, s, kp, g, G.item0, G.item1, F, res);
         });
     if (!overlap)
         pipe.fork();

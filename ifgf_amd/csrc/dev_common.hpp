@@ -729,16 +729,16 @@ static inline void set_xcd_log2(int lg)
     h_xcd_log2 = lg;
 }
 
-// The near field and the far levels of a mult on the plan's side stream (host side).  The
+// The near field and the far groups of a mult on the plan's side stream (host side).  The
 // near field only writes the result (after the zeroing) and the far field adds to it (after
-// the near field), one target level after another, deepest first; the far field of a target
-// level reads the coefficients of its source levels only.  With `on`, fork() starts the near
-// field on the side stream (called after the deepest level's transform, so that it runs beside
-// the memory-bound upward kernels rather than the VALU-bound leaf sampling), ready(l), called
-// once the upward chain has written source level l, enqueues there every far level whose
-// shallowest source level is l or deeper, and finish() joins the side stream back.  Without
-// `on` the near field runs at fork() and the far levels at finish(), in the same order, on the
-// main stream (the sums are the same).
+// the near field), one far group after another (a group: one source level and one target
+// level, deepest source level first), each reading the coefficients of its source level only.
+// With `on`, fork() starts the near field on the side stream (called after the deepest level's
+// transform, so that it runs beside the memory-bound upward kernels rather than the VALU-bound
+// leaf sampling), ready(l), called once the upward chain has written source level l, enqueues
+// there the far groups of level l (and any deeper ones left), and finish() joins the side
+// stream back.  Without `on` the near field runs at fork() and the far groups at finish(), in
+// the same order, on the main stream (the sums are the same).
 template <class NearFn, class FarFn> struct SidePipe
 {
     DevicePlan& D;
@@ -747,10 +747,10 @@ template <class NearFn, class FarFn> struct SidePipe
     NearFn near; // near(stream)
     FarFn far;   // far(target level, stream)
     bool forked = false;
-    int next;    // the next far level to enqueue (deepest first)
+    int next;    // the next far group to enqueue (deepest source level first)
     SidePipe(DevicePlan& d, hipStream_t s, bool overlap, NearFn n, FarFn f)
         : D(d), st(s), on(overlap && d.side_stream), near(n), far(f),
-          next(int(d.far_item_off.size()) - 2)
+          next(0)
     {
     }
     hipStream_t side() const { return static_cast<hipStream_t>(D.side_stream); }
@@ -773,7 +773,8 @@ template <class NearFn, class FarFn> struct SidePipe
         if (!on || !forked)
             return;
         bool waited = false;
-        while (next >= 0 && D.far_min_src[next] >= done)
+        const int ng = int(D.far_groups.size());
+        while (next < ng && D.far_groups[next].src_level >= done)
         {
             if (!waited)
             {
@@ -782,7 +783,7 @@ template <class NearFn, class FarFn> struct SidePipe
                 HIP_OK(hipStreamWaitEvent(side(), e, 0));
                 waited = true;
             }
-            far(next--, side());
+            far(next++, side());
         }
     }
     void finish()
@@ -795,7 +796,7 @@ template <class NearFn, class FarFn> struct SidePipe
             HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_near), 0));
             return;
         }
-        for (; next >= 0; next--)
+        for (; next < int(D.far_groups.size()); next++)
             far(next, st);
     }
 };

@@ -571,7 +571,7 @@ template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int WPB>
 __global__ __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) void k_far(const WorkItem* __restrict__ items,
                                                  int64_t nitems,
-                                            const int32_t* __restrict__ far_off,
+                                            const int32_t* __restrict__ srange,
                                             const int32_t* __restrict__ far_src,
                                             const BoxGeo* __restrict__ geo,
                                             const CellRef* __restrict__ ctab,
@@ -604,7 +604,7 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
     const int64_t t = int64_t(wi.first) + (valid ? ti : 0);
     const double y0 = tpts[3 * t], y1 = tpts[3 * t + 1], y2 = tpts[3 * t + 2];
     Cx<R> acc = {0, 0};
-    const int32_t j0 = far_off[wi.box], j1 = far_off[wi.box + 1];
+    const int32_t j0 = srange[2 * it], j1 = srange[2 * it + 1];
     auto body = [&](const BoxGeo& B)
     {
         int fine;
@@ -744,13 +744,14 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     }
     else
     {
-        const int abase = D.far_ang_base[far_level], an = D.far_ang_n[far_level];
+        // far_level: the far group; a, b: its items
+        const int abase = D.far_groups[far_level].ang_base, an = D.far_groups[far_level].ang_n;
         const int64_t n = b - a;
 #define IFGF_FAR(FAST, WPB, LDS, BASE, AN)                                                      \
     k_far<PH, R, O0, O1, O2, FAST, WPB>                                                         \
         <<<dim3(unsigned((n + WPB - 1) / WPB)), dim3(64 * WPB), LDS, st>>>(                     \
-            D.far_items + a, n, D.far_off, D.far_src, D.geo, D.ctab, F, D.tgt_pts, D.fine_ang,  \
-            BASE, AN, res, kp)
+            D.far_items + a, n, D.far_srange + 2 * a, D.far_src, D.geo, D.ctab, F, D.tgt_pts,   \
+            D.fine_ang, BASE, AN, res, kp)
         if constexpr (sizeof(R) == 4)
             if (an > 0)
             {
@@ -964,11 +965,10 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
                     D.near_items, D.n_near_items, D.near_off, D.near_src, D.geo, D.src_begin,
                     D.src_end, D.src_pts, D.normals, ws, D.tgt_pts, res, kp);
         },
-        [&](int l, hipStream_t s)
+        [&](int g, hipStream_t s)
         {
-            const int64_t a = D.far_item_off[l], b = D.far_item_off[l + 1];
-            if (b > a)
-                launch_up_far<PH, R>(D, s, kp, tabs, -1, l, a, b, F, V, res);
+            const DevicePlan::FarGroup& G = D.far_groups[g];
+            launch_up_far<PH, R>(D, s, kp, tabs, -1, g, G.item0, G.item1, F, V, res);
         });
     if (!overlap)
         pipe.fork(); // the near field first, in order (and inside the timed near phase)
@@ -1087,7 +1087,7 @@ void device_free(DevicePlan& D)
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
                     D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_glist,
-                    D.leaf_cones, D.up_cones, D.far_off, D.far_src,
+                    D.leaf_cones, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
     for (void* p : ptrs)
@@ -1410,46 +1410,58 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         std::vector<int32_t> foff, fsrc, ftb, noff, nsrcv, ntb;
         csr(P.far, foff, fsrc, ftb);
         csr(P.near, noff, nsrcv, ntb);
-        D.far_off = dupload(foff, tot);
         D.far_src = dupload(fsrc, tot);
         D.near_off = dupload(noff, tot);
         D.near_src = dupload(nsrcv, tot);
-        // far work items by target level (one launch per level: no target is written twice)
+        // far work: items of at most 64 targets of one target box against its far sources of
+        // one source level (contiguous in its list: box ids are level-major), grouped by (source
+        // level, target level), deepest source level first, then deepest target level
         std::vector<WorkItem> fitems;
-        D.far_item_off.assign(1, 0);
-        D.far_ang_base.clear();
-        D.far_ang_n.clear();
-        D.far_min_src.clear();
-        for (int l = 0; l < P.T.levels; l++)
-        {
-            std::vector<int32_t> bl;
-            for (int32_t t : ftb)
-                if (t >= P.T.level_off[l] && t < P.T.level_off[l + 1])
-                    bl.push_back(t);
-            auto it = make_items(bl, P.T);
-            fitems.insert(fitems.end(), it.begin(), it.end());
-            D.far_item_off.push_back(int64_t(fitems.size()));
-            // fine-grid angle tables of the level's far sources (boxes of several levels: the
-            // dual traversal pairs a target level with deeper and shallower source boxes): the
-            // range of the angle array covering all their grid classes, staged in LDS by the
-            // mixed far kernel when it fits 48 KB
-            int64_t lo = INT64_MAX, hi = -1;
-            for (int32_t t : bl)
-                for (int32_t j = foff[t]; j < foff[t + 1]; j++)
+        std::vector<int32_t> frange;
+        D.far_groups.clear();
+        std::vector<std::vector<int32_t>> tb_of_level(P.T.levels);
+        for (int32_t t : ftb)
+            tb_of_level[P.T.level_of(t)].push_back(t);
+        for (int sl = P.S.levels - 1; sl >= 0; sl--)
+            for (int tl = P.T.levels - 1; tl >= 0; tl--)
+            {
+                const int64_t i0 = int64_t(fitems.size());
+                int64_t lo = INT64_MAX, hi = -1;
+                for (int32_t t : tb_of_level[tl])
                 {
-                    const BoxGeo& B = geo[fsrc[j]];
-                    lo = std::min<int64_t>(lo, 2 * int64_t(B.fth_off));
-                    hi = std::max<int64_t>(hi, 2 * int64_t(B.fph_off) + 2 * (2 * B.nf[2] + 1));
+                    // the sources ascend by box id, so by level: the level-sl run
+                    int32_t j0 = foff[t];
+                    const int32_t j1 = foff[t + 1];
+                    while (j0 < j1 && P.S.level_of(fsrc[j0]) < sl)
+                        j0++;
+                    int32_t j = j0;
+                    while (j < j1 && P.S.level_of(fsrc[j]) == sl)
+                        j++;
+                    if (j == j0)
+                        continue;
+                    for (int32_t m = j0; m < j; m++)
+                    {
+                        const BoxGeo& B = geo[fsrc[m]];
+                        lo = std::min<int64_t>(lo, 2 * int64_t(B.fth_off));
+                        hi = std::max<int64_t>(hi, 2 * int64_t(B.fph_off) + 2 * (2 * B.nf[2] + 1));
+                    }
+                    const int64_t s0 = P.T.begin[t], e0 = P.T.end[t];
+                    for (int64_t f = s0; f < e0; f += 64)
+                    {
+                        fitems.push_back({t, int32_t(f), int32_t(std::min<int64_t>(64, e0 - f)), 0});
+                        frange.push_back(j0);
+                        frange.push_back(j);
+                    }
                 }
-            const bool fits = hi > lo && (hi - lo) * int64_t(sizeof(double)) <= 48 * 1024;
-            D.far_ang_base.push_back(fits ? int(lo) : 0);
-            D.far_ang_n.push_back(fits ? int(hi - lo) : 0);
-            int min_src = P.S.levels - 1;
-            for (int32_t t : bl)
-                for (int32_t j = foff[t]; j < foff[t + 1]; j++)
-                    min_src = std::min(min_src, P.S.level_of(fsrc[j]));
-            D.far_min_src.push_back(min_src);
-        }
+                if (int64_t(fitems.size()) == i0)
+                    continue;
+                // the angle tables of the group's grid classes, staged in LDS by the mixed far
+                // kernel when they fit 48 KB
+                const bool fits = hi > lo && (hi - lo) * int64_t(sizeof(double)) <= 48 * 1024;
+                D.far_groups.push_back({i0, int64_t(fitems.size()), fits ? int(lo) : 0,
+                                        fits ? int(hi - lo) : 0, sl, tl});
+            }
+        D.far_srange = dupload(frange, tot);
         D.far_items = dupload(fitems, tot);
         auto nitems = make_items(ntb, P.T);
         D.near_items = dupload(nitems, tot);

@@ -126,14 +126,21 @@ struct DevicePlan
     std::vector<int64_t> up_off;             // per level, into up_cones
     std::vector<int64_t> cone_lvl_off;       // per level, global cone range
 
-    int32_t* far_off = nullptr;   // per target box: CSR into far_src
-    int32_t* far_src = nullptr;
+    int32_t* far_src = nullptr;   // far source boxes, by target box then source box
+    int32_t* far_srange = nullptr; // per far item: its sources far_src[2i] .. far_src[2i + 1]
     int32_t* near_off = nullptr;
     int32_t* near_src = nullptr;
     WorkItem* far_items = nullptr;
-    std::vector<int64_t> far_item_off; // per target level
-    std::vector<int> far_ang_base, far_ang_n; // per target level: angle tables staged in LDS
-    std::vector<int> far_min_src; // per target level: the shallowest source level it reads
+    // far work in launch groups of one source level and one target level, deepest source level
+    // first: a group can run as soon as its source level's coefficients exist; groups run one
+    // after another (a target point is in boxes of several levels)
+    struct FarGroup
+    {
+        int64_t item0, item1; // far_items[item0 .. item1)
+        int ang_base, ang_n;  // the sources' angle tables staged in LDS (mixed form; 0: none)
+        int src_level, tgt_level;
+    };
+    std::vector<FarGroup> far_groups;
     WorkItem* near_items = nullptr;
     int64_t n_near_items = 0;
 
