@@ -333,6 +333,73 @@ __global__ __launch_bounds__(256) void k_up_cell_b(const WorkItem* __restrict__ 
     }
 }
 
+// LDS cache of the coefficient blocks (N x RB: the N coefficients of one fine cone for the RB
+// columns) of up to CACHE distinct fine cones (slots), shared by a workgroup: clear(), then
+// every thread insert()s its slot (an open-addressing table in LDS, HASH entries; -1 when
+// the table is full), assign() numbers the distinct slots, load() copies their blocks, and
+// block() gives a slot's coefficients: in LDS when cached, else in memory (the same values).
+// Each step is separated from the next by a workgroup barrier.
+template <typename R, int N, int RB, int CACHE, int HASH> struct SlotCache
+{
+    Cx<R>* coef;      // CACHE * N * RB
+    int32_t* hash;    // HASH: slot or -1
+    int32_t* cidx;    // HASH: cache index or -1
+    int32_t* cached;  // CACHE: slots
+    int32_t* ncache;
+    __device__ void clear(int tid) const
+    {
+        if (tid < HASH)
+            hash[tid] = -1;
+        if (tid == 0)
+            *ncache = 0;
+    }
+    __device__ int insert(int slot) const
+    {
+        if (slot < 0)
+            return -1;
+        int at = slot & (HASH - 1);
+        for (int probe = 0; probe < HASH; probe++)
+        {
+            const int32_t prev = atomicCAS(&hash[at], -1, slot);
+            if (prev == -1 || prev == slot)
+                return at;
+            at = (at + 1) & (HASH - 1);
+        }
+        return -1;
+    }
+    __device__ void assign(int tid) const
+    {
+        if (tid < HASH)
+        {
+            const int32_t sl = hash[tid];
+            int ci = -1;
+            if (sl >= 0)
+            {
+                ci = atomicAdd(ncache, 1);
+                if (ci < CACHE)
+                    cached[ci] = sl;
+                else
+                    ci = -1;
+            }
+            cidx[tid] = ci;
+        }
+    }
+    __device__ void load(const Cx<R>* __restrict__ F, int tid, int nthreads) const
+    {
+        const int nc = min(*ncache, CACHE);
+        for (int e = tid; e < nc * N * RB; e += nthreads)
+        {
+            const int b = e / (N * RB), o = e - b * (N * RB);
+            coef[e] = F[int64_t(cached[b]) * N * RB + o];
+        }
+    }
+    __device__ const Cx<R>* block(const Cx<R>* F, int slot, int h) const
+    {
+        const int ci = h >= 0 ? cidx[h] : -1;
+        return ci >= 0 ? coef + ci * N * RB : F + int64_t(slot) * N * RB;
+    }
+};
+
 // upward pass with tables, order 6 (the batched form of k_up_cell with the children's
 // coefficients staged in LDS).  One workgroup per item (cell, up to K parent cones of that
 // cell); for each cone and each child in order: the nodes' fine cones in the child (table
@@ -361,10 +428,9 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
     constexpr int NQ = (NH + TQ - 1) / TQ;   // nodes per thread
     static_assert(NH <= 256, "one thread per node while collecting");
     __shared__ Cx<R> s_coef[UPS_CACHE * N * RB];
-    __shared__ int32_t s_hash[UPS_HASH];     // fine cone (slot) or -1
-    __shared__ int32_t s_cidx[UPS_HASH];     // its cache index or -1
-    __shared__ int32_t s_cached[UPS_CACHE];  // slots in the cache
-    __shared__ int32_t s_ncache;
+    __shared__ int32_t s_hash[UPS_HASH], s_cidx[UPS_HASH], s_cached[UPS_CACHE], s_ncache;
+    const SlotCache<R, N, RB, UPS_CACHE, UPS_HASH> cache{s_coef, s_hash, s_cidx, s_cached,
+                                                          &s_ncache};
     __shared__ int32_t s_qslot[NH];          // fine cone of each node in this child (or -1)
     __shared__ int32_t s_qh[NH];             // its hash position
     __shared__ R s_qx[3][NH];
@@ -390,10 +456,7 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
             const int32_t ch = child[8 * pbox + c];
             if (ch < 0)
                 continue; // uniform: the same child for every thread
-            if (tid < UPS_HASH)
-                s_hash[tid] = -1;
-            if (tid == 0)
-                s_ncache = 0;
+            cache.clear(tid);
             __syncthreads();
             // collect: one thread per node
             if (tid < NH)
@@ -406,51 +469,17 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
                     if ((cr.mask >> E.sub) & 1u)
                         slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
                 }
-                int h = -1; // hash position of the slot; -1: not in the table (read from memory)
-                if (slot >= 0)
-                {
-                    int at = slot & (UPS_HASH - 1);
-                    for (int probe = 0; probe < UPS_HASH; probe++)
-                    {
-                        const int32_t prev = atomicCAS(&s_hash[at], -1, slot);
-                        if (prev == -1 || prev == slot)
-                        {
-                            h = at;
-                            break;
-                        }
-                        at = (at + 1) & (UPS_HASH - 1);
-                    }
-                }
                 s_qslot[tid] = slot;
-                s_qh[tid] = h;
+                s_qh[tid] = cache.insert(slot);
                 s_qx[0][tid] = E.xi0;
                 s_qx[1][tid] = E.xi1;
                 s_qx[2][tid] = E.xi2;
                 s_qtf[tid] = Cx<R>{E.tr, E.ti};
             }
             __syncthreads();
-            // cache indices of the distinct slots (table order)
-            if (tid < UPS_HASH)
-            {
-                const int32_t sl = s_hash[tid];
-                int ci = -1;
-                if (sl >= 0)
-                {
-                    ci = atomicAdd(&s_ncache, 1);
-                    if (ci < UPS_CACHE)
-                        s_cached[ci] = sl;
-                    else
-                        ci = -1;
-                }
-                s_cidx[tid] = ci;
-            }
+            cache.assign(tid);
             __syncthreads();
-            const int nc = min(s_ncache, UPS_CACHE);
-            for (int e = tid; e < nc * N * RB; e += 256)
-            {
-                const int b = e / (N * RB), o = e - b * (N * RB);
-                s_coef[e] = F[int64_t(s_cached[b]) * N * RB + o];
-            }
+            cache.load(F, tid, 256);
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < NQ; j++)
@@ -461,9 +490,7 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
                 const int slot = s_qslot[q];
                 if (slot < 0)
                     continue;
-                const int h = s_qh[q];
-                const int ci = h >= 0 ? s_cidx[h] : -1;
-                const Cx<R>* blk = ci >= 0 ? s_coef + ci * N * RB : F + int64_t(slot) * N * RB;
+                const Cx<R>* blk = cache.block(F, slot, s_qh[q]);
                 Cx<R> cf[N];
 #pragma unroll
                 for (int i = 0; i < N; i++)
