@@ -864,7 +864,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         w_dev, D.src_perm, ws, D.nsrc, RB, ncol, in_im);
     k_zero_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(res, nt);
     // near and far fields as in the single-column mult (SidePipe)
-    const bool overlap = D.shape.overlap && D.n_near_items;
+    const int overlap = D.n_near_items ? D.shape.overlap : 0;
     auto pipe = side_pipe(
         D, st, overlap,
         [&](hipStream_t s)
@@ -879,7 +879,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
             const DevicePlan::FarGroup& G = D.far_groups[g];
             launch_far_b<PH, R, O0, O1, O2, RB>(D, s, kp, g, G.item0, G.item1, F, res);
         });
-    if (!overlap)
+    if (overlap != 1)
         pipe.fork();
     if (D.n_leaf_cones)
         k_leaf_b<KIND, R, RB><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(

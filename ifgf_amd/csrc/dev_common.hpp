@@ -733,24 +733,26 @@ static inline void set_xcd_log2(int lg)
 // near field only writes the result (after the zeroing) and the far field adds to it (after
 // the near field), one far group after another (a group: one source level and one target
 // level, deepest source level first), each reading the coefficients of its source level only.
-// With `on`, fork() starts the near field on the side stream (called after the deepest level's
+// mode 1: fork() starts the near field on the side stream (called after the deepest level's
 // transform, so that it runs beside the memory-bound upward kernels rather than the VALU-bound
-// leaf sampling), ready(l), called once the upward chain has written source level l, enqueues
-// there the far groups of level l (and any deeper ones left), and finish() joins the side
-// stream back.  Without `on` the near field runs at fork() and the far groups at finish(), in
-// the same order, on the main stream (the sums are the same).
+// leaf sampling); mode 2: fork() runs it on the main stream (called first).  In both, ready(l),
+// called once the upward chain has written source level l, enqueues on the side stream the far
+// groups of level l (and any deeper ones left), and finish() joins the side stream back.  Mode
+// 0: the near field at fork() and the far groups at finish(), in the same order, on the main
+// stream (the sums are the same in every mode).
 template <class NearFn, class FarFn> struct SidePipe
 {
     DevicePlan& D;
     hipStream_t st;
-    bool on;
+    bool on;     // far groups on the side stream (modes 1 and 2)
+    bool near_side; // the near field on the side stream (mode 1)
     NearFn near; // near(stream)
     FarFn far;   // far(target level, stream)
     bool forked = false;
     int next;    // the next far group to enqueue (deepest source level first)
-    SidePipe(DevicePlan& d, hipStream_t s, bool overlap, NearFn n, FarFn f)
-        : D(d), st(s), on(overlap && d.side_stream), near(n), far(f),
-          next(0)
+    SidePipe(DevicePlan& d, hipStream_t s, int mode, NearFn n, FarFn f)
+        : D(d), st(s), on(mode != 0 && d.side_stream), near_side(on && mode == 1), near(n),
+          far(f), next(0)
     {
     }
     hipStream_t side() const { return static_cast<hipStream_t>(D.side_stream); }
@@ -759,7 +761,7 @@ template <class NearFn, class FarFn> struct SidePipe
         if (forked)
             return;
         forked = true;
-        if (!on)
+        if (!near_side)
         {
             near(st);
             return;
@@ -801,9 +803,9 @@ template <class NearFn, class FarFn> struct SidePipe
     }
 };
 template <class NearFn, class FarFn>
-SidePipe<NearFn, FarFn> side_pipe(DevicePlan& D, hipStream_t st, bool overlap, NearFn n, FarFn f)
+SidePipe<NearFn, FarFn> side_pipe(DevicePlan& D, hipStream_t st, int mode, NearFn n, FarFn f)
 {
-    return SidePipe<NearFn, FarFn>(D, st, overlap, n, f);
+    return SidePipe<NearFn, FarFn>(D, st, mode, n, f);
 }
 
 __device__ __forceinline__ int pow2_at_least(int n)

@@ -758,17 +758,17 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
                 const size_t lds = size_t(an) * sizeof(double);
                 switch (D.shape.far_waves)
                 {
-                case 1:
-                    IFGF_FAR(true, 1, lds, abase, an);
-                    break;
                 case 2:
                     IFGF_FAR(true, 2, lds, abase, an);
                     break;
                 case 8:
                     IFGF_FAR(true, 8, lds, abase, an);
                     break;
-                default:
+                case 4:
                     IFGF_FAR(true, 4, lds, abase, an);
+                    break;
+                default:
+                    IFGF_FAR(true, 1, lds, abase, an);
                 }
                 return;
             }
@@ -953,7 +953,7 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
     // near and far fields: SidePipe (overlapped with the upward chain on the side stream when
     // the shape asks for it; phase timing runs everything in order on one stream)
-    const bool overlap = D.shape.overlap && !pt.on && D.n_near_items;
+    const int overlap = (pt.on || !D.n_near_items) ? 0 : D.shape.overlap;
     pt.begin(0);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
     auto pipe = side_pipe(
@@ -970,8 +970,8 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
             const DevicePlan::FarGroup& G = D.far_groups[g];
             launch_up_far<PH, R>(D, s, kp, tabs, -1, g, G.item0, G.item1, F, V, res);
         });
-    if (!overlap)
-        pipe.fork(); // the near field first, in order (and inside the timed near phase)
+    if (overlap != 1)
+        pipe.fork(); // the near field first, on the main stream (inside the timed near phase)
     pt.end();
     pt.begin(1);
     if (D.n_leaf_cones)
@@ -1633,8 +1633,8 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
         return "launch shape: batched columns per pass must be 8 or 32";
     if (sh.far_waves != 1 && sh.far_waves != 2 && sh.far_waves != 4 && sh.far_waves != 8)
         return "launch shape: far-field waves per workgroup must be 1, 2, 4 or 8";
-    if (sh.overlap != 0 && sh.overlap != 1)
-        return "launch shape: overlap is 0 or 1";
+    if (sh.overlap < 0 || sh.overlap > 2)
+        return "launch shape: overlap is 0, 1 or 2";
     if (sh.cpb != 0)
     {
         const auto ch = transform_cpb_choices(order_class(D), D.prec == PREC_FP64 ? 8 : 4);
@@ -1742,8 +1742,7 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
         pick(&LaunchShape::up_k, {4, 8, 16}, batched ? -1 : 2);
         if (!batched && D.prec == PREC_MIXED)
             pick(&LaunchShape::far_waves, {1, 2, 4, 8}, 4);
-        if (!batched)
-            pick(&LaunchShape::overlap, {0, 1}, -1);
+        pick(&LaunchShape::overlap, {0, 1, 2}, -1);
         if (batched && nrhs > 8)
             pick(&LaunchShape::rb, {8, 32}, -1);
         err = device_set_shape(D, cur);

@@ -67,9 +67,9 @@ struct LaunchShape
     int cpb = 0;      // transform: cones per workgroup (0: the order class's default)
     int up_k = 16;    // upward: parent cones of one cell per workgroup (4, 8 or 16)
     int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
-    int far_waves = 4; // mixed far field: waves (items) per workgroup sharing the LDS tables
-    int overlap = 1;   // single-column mult: near field on a second stream beside the upward
-                       // pass (VALU-bound beside memory-bound work), joined before the far field
+    int far_waves = 1; // mixed far field: waves (items) per workgroup sharing the LDS tables
+    int overlap = 1;   // far field on a second stream beside the upward chain, with the near
+                       // field there too (1) or first on the main stream (2); 0: one stream
 };
 constexpr int UP_K_CHOICES[3] = {4, 8, 16};
 

@@ -138,7 +138,8 @@ int ifgf_last_timings(const ifgf_operator* op, double* ms6);
  *         [2] upward parent cones per workgroup (4, 8, 16; default 16)
  *         [3] batched columns per pass (0 = by nrhs, 8, 32)
  *         [4] mixed far field: waves per workgroup sharing the staged angle tables (1, 2, 4, 8)
- *         [5] one column: near field on a second stream beside the upward pass (0, 1)
+ *         [5] second stream for the far field beside the upward chain, with the near field
+ *             there (1) or first on the main stream (2); 0: one stream
  * Shapes change speed, never results.  ifgf_autotune times the candidates on the operator
  * itself (synthetic densities, nrhs columns, the fastest of reps mults per candidate), one
  * choice after another, applies the best and returns it; after ifgf_init, not thread-safe

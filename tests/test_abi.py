@@ -43,7 +43,7 @@ def test_version_and_errors_without_device():
     assert L.ifgf_autotune(h, 1, 1, shape.ctypes.data) == _lib.IFGF_ESTATE
     assert L.ifgf_set_launch_shape(h, shape.ctypes.data) == _lib.IFGF_ESTATE
     assert L.ifgf_get_launch_shape(h, shape.ctypes.data) == _lib.IFGF_OK
-    assert shape.tolist() == [4, 0, 16, 0, 4, 1]  # the defaults
+    assert shape.tolist() == [4, 0, 16, 0, 1, 1]  # the defaults
     if L.ifgf_device_count() == 0:
         x = np.random.default_rng(0).standard_normal((50, 3))
         rc = L.ifgf_init(h, x.ctypes.data, 50, x.ctypes.data, 50, None, _lib.CUTOFF_FN(0), None, 0)

@@ -24,7 +24,7 @@ def test_every_shape_gives_the_same_result(gpu, prec, order):
     ref = op.mult(w)
     waves = [1, 2, 4, 8] if prec == "mixed" else [4]
     for lg, cpb, k, fw, ov in itertools.product([0, 2, 6], CPB[(prec, order)], [4, 8, 16], waves,
-                                                [0, 1]):
+                                                [0, 1, 2]):
         op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k, far_waves=fw,
                                overlap=ov)
         print(f"shape prec={prec} order={order} xcd_log2={lg} cpb={cpb} up_k={k} far_waves={fw} "
