@@ -139,38 +139,45 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
         result[t] = cadd(result[t], acc);
 }
 
-// leaf sampling: factored kernel of the leaf's sources at the high-order nodes of one
-// active coarse cone (ifgfoperator.hpp:1057-1073, :423-462)
+// leaf sampling (ifgfoperator.hpp:1057-1073, :423-462): the factored kernel of a leaf's
+// sources at the high-order nodes of its active coarse cones.  One 256-thread workgroup per
+// leaf box, thread = (active cone,
+// high-order node) pairs of the box, 256 at a time, the box's sources staged in LDS for each
+// round; lanes stay busy however many cones the box has (per-cone blocks of 128 threads leave
+// 20 of them idle with 108 nodes), and the sources are staged once per 256 pairs
 template <int KIND, typename R>
-__global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, int64_t nwork,
-                                              const ConeDev* __restrict__ cones,
-                                              const BoxGeo* __restrict__ geo,
-                                              const int64_t* __restrict__ sbeg,
-                                              const int64_t* __restrict__ send,
-                                              const double* __restrict__ spts,
-                                              const double* __restrict__ snrm,
-                                              const Cx<R>* __restrict__ w, NodeTabs tabs,
-                                              Cx<R>* __restrict__ V, KParams kp)
+__global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ items,
+                                                  int64_t nitems,
+                                                  const ConeDev* __restrict__ cones,
+                                                  const BoxGeo* __restrict__ geo,
+                                                  const int64_t* __restrict__ sbeg,
+                                                  const int64_t* __restrict__ send,
+                                                  const double* __restrict__ spts,
+                                                  const double* __restrict__ snrm,
+                                                  const Cx<R>* __restrict__ w, NodeTabs tabs,
+                                                  Cx<R>* __restrict__ V, KParams kp)
 {
-    constexpr int CH = 128;
+    constexpr int CH = 256;
     __shared__ R sx[CH], sy[CH], sz[CH];
     __shared__ R nx[CH], ny[CH], nz[CH];
     __shared__ Cx<R> sw[CH];
-    const int64_t wi = xcd_block();
-    if (wi >= nwork)
+    const int64_t it = xcd_block();
+    if (it >= nitems)
         return;
-    const int32_t g = work[wi];
-    const ConeDev cn = cones[g];
-    const BoxGeo B = geo[cn.box];
+    const WorkItem wi = items[it]; // box, its first cone, its number of cones
+    const BoxGeo B = geo[wi.box];
     const KParams kb = with_k(kp, B);
-    const int64_t b0 = sbeg[cn.box], b1 = send[cn.box];
-    for (int q0 = 0; q0 < kp.n_hi; q0 += CH)
+    const int64_t b0 = sbeg[wi.box], b1 = send[wi.box];
+    const int npair = wi.count * kp.n_hi;
+    for (int p0 = 0; p0 < npair; p0 += CH)
     {
-        const int q = q0 + threadIdx.x;
-        const bool active = q < kp.n_hi;
+        const int e = p0 + int(threadIdx.x);
+        const bool active = e < npair;
+        const int k = active ? e / kp.n_hi : 0, q = active ? e - k * kp.n_hi : 0;
+        const int32_t g = wi.first + k;
         double yd[3] = {0, 0, 0}, r = 1;
         if (active)
-            cone_node(B, cn.cell, q, kp.ho0, kp.ho1, kp.ho2, tabs, false, yd, r);
+            cone_node(B, cones[g].cell, q, kp.ho0, kp.ho1, kp.ho2, tabs, false, yd, r);
         const R y0 = R(yd[0]), y1 = R(yd[1]), y2 = R(yd[2]), dc = R(r);
         Cx<R> acc = {0, 0};
         for (int64_t c0 = b0; c0 < b1; c0 += CH)
@@ -193,17 +200,17 @@ __global__ __launch_bounds__(128) void k_leaf(const int32_t* __restrict__ work, 
             __syncthreads();
             const int n = int(min<int64_t>(CH, b1 - c0));
             if (active)
-                for (int k = 0; k < n; k++)
+                for (int m = 0; m < n; m++)
                 {
                     R nn[3] = {0, 0, 0};
                     if (snrm)
                     {
-                        nn[0] = nx[k];
-                        nn[1] = ny[k];
-                        nn[2] = nz[k];
+                        nn[0] = nx[m];
+                        nn[1] = ny[m];
+                        nn[2] = nz[m];
                     }
-                    acc = cadd(acc, factored<KIND, R>(kb, y0, y1, y2, dc, sx[k], sy[k], sz[k], nn,
-                                                      sw[k]));
+                    acc = cadd(acc, factored<KIND, R>(kb, y0, y1, y2, dc, sx[m], sy[m], sz[m], nn,
+                                                      sw[m]));
                 }
         }
         if (active)
@@ -974,9 +981,9 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
         pipe.fork(); // the near field first, on the main stream (inside the timed near phase)
     pt.end();
     pt.begin(1);
-    if (D.n_leaf_cones)
-        k_leaf<KIND, R><<<dim3(unsigned(D.n_leaf_cones)), dim3(128), 0, st>>>(
-            D.leaf_cones, D.n_leaf_cones, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
+    if (D.n_leaf_boxes)
+        k_leaf_box<KIND, R><<<dim3(unsigned(D.n_leaf_boxes)), dim3(256), 0, st>>>(
+            D.leaf_boxes, D.n_leaf_boxes, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
             D.normals, ws, tabs, V, kp);
     pt.end();
     for (int l = D.levels - 1; l >= 0; l--)
@@ -1087,7 +1094,7 @@ void device_free(DevicePlan& D)
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
                     D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_glist,
-                    D.leaf_cones, D.up_cones, D.far_srange, D.far_src,
+                    D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
     for (void* p : ptrs)
@@ -1387,6 +1394,16 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         D.up_glist = dupload(glist, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
+        {
+            // leaf boxes with active cones: (box, first cone, number of cones)
+            std::vector<WorkItem> lb;
+            for (int64_t g = 0; g < P.S.num_boxes(); g++)
+                if (P.S.leaf[g] && P.cone_off[g + 1] > P.cone_off[g])
+                    lb.push_back({int32_t(g), int32_t(P.cone_off[g]),
+                                  int32_t(P.cone_off[g + 1] - P.cone_off[g]), 0});
+            D.leaf_boxes = dupload(lb, tot);
+            D.n_leaf_boxes = int64_t(lb.size());
+        }
         D.up_cones = dupload(up, tot);
         // far / near lists grouped by target box (pairs are sorted by target)
         const int64_t nt = P.T.num_boxes();

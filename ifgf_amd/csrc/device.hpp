@@ -122,6 +122,8 @@ struct DevicePlan
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
+    WorkItem* leaf_boxes = nullptr; // leaf boxes with active cones: (box, first cone, count)
+    int64_t n_leaf_boxes = 0;
     int32_t* up_cones = nullptr;             // non-leaf cones without a table, level-major
     std::vector<int64_t> up_off;             // per level, into up_cones
     std::vector<int64_t> cone_lvl_off;       // per level, global cone range
