@@ -141,10 +141,9 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
 
 // leaf sampling (ifgfoperator.hpp:1057-1073, :423-462): the factored kernel of a leaf's
 // sources at the high-order nodes of its active coarse cones.  One 256-thread workgroup per
-// leaf box, thread = (active cone,
-// high-order node) pairs of the box, 256 at a time, the box's sources staged in LDS for each
-// round; lanes stay busy however many cones the box has (per-cone blocks of 128 threads leave
-// 20 of them idle with 108 nodes), and the sources are staged once per 256 pairs
+// leaf box, thread = (active cone, high-order node) pairs of the box, 256 at a time, the box's
+// sources staged in LDS for each round: lanes stay busy however many cones the box has, and
+// the sources are staged once per 256 pairs
 template <int KIND, typename R>
 __global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ items,
                                                   int64_t nitems,
