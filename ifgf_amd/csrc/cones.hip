@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "hip_util.hpp"
 #include "plan.hpp"
 
 #pragma clang fp contract(off)
@@ -29,14 +30,6 @@ namespace ifgf
 {
 namespace
 {
-#define CONES_HIP_OK(x)                                                                    \
-    do                                                                                     \
-    {                                                                                      \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
 // one source box of the level: centre, side, the shared cone domain and both grids
 struct MarkBox
 {
@@ -199,39 +192,6 @@ __global__ __launch_bounds__(256) void k_mark_parent(
     }
 }
 
-struct DBuf
-{
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() { return static_cast<T*>(p); }
-    void reserve(size_t b)
-    {
-        if (b <= bytes)
-            return;
-        release();
-        CONES_HIP_OK(hipMalloc(&p, std::max<size_t>(b, 256)));
-        bytes = std::max<size_t>(b, 256);
-    }
-};
-
-template <class T> void h2d(DBuf& d, const T* h, size_t count)
-{
-    d.reserve(count * sizeof(T));
-    if (count)
-        CONES_HIP_OK(hipMemcpy(d.p, h, count * sizeof(T), hipMemcpyHostToDevice));
-}
-
 // device buffers kept across the levels of one build_cones
 struct MarkerState
 {
@@ -323,9 +283,9 @@ void mark_level(MarkerState& st, const Plan& P, int level, const std::vector<int
     st.fbits.reserve(sizeof(uint64_t) * lb.fbits.size());
     st.cbits.reserve(sizeof(uint64_t) * lb.cbits.size());
     if (!lb.fbits.empty())
-        CONES_HIP_OK(hipMemset(st.fbits.p, 0, sizeof(uint64_t) * lb.fbits.size()));
+        HIP_OK(hipMemset(st.fbits.p, 0, sizeof(uint64_t) * lb.fbits.size()));
     if (!lb.cbits.empty())
-        CONES_HIP_OK(hipMemset(st.cbits.p, 0, sizeof(uint64_t) * lb.cbits.size()));
+        HIP_OK(hipMemset(st.cbits.p, 0, sizeof(uint64_t) * lb.cbits.size()));
     if (r1 > r0)
     {
         h2d(st.pair_box, pair_box.data(), pair_box.size());
@@ -333,7 +293,7 @@ void mark_level(MarkerState& st, const Plan& P, int level, const std::vector<int
             st.boxes.as<MarkBox>(), st.pair_box.as<int32_t>(), st.far_tgt.as<int32_t>() + r0,
             st.tbeg.as<int64_t>(), st.tend.as<int64_t>(), st.tpts.as<double>(),
             st.fbits.as<unsigned long long>(), st.cbits.as<unsigned long long>());
-        CONES_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
     }
     if (!work.empty())
     {
@@ -362,13 +322,13 @@ void mark_level(MarkerState& st, const Plan& P, int level, const std::vector<int
             st.boxes.as<MarkBox>(), st.work.as<ParentWork>(), st.tab_fc.as<int32_t>(),
             st.cone_u.as<int32_t>(), st.cone_cell.as<int64_t>(), st.ho.as<double>(), n_ho,
             table ? 1 : 0, st.fbits.as<unsigned long long>(), st.cbits.as<unsigned long long>());
-        CONES_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
     }
     if (!lb.fbits.empty())
-        CONES_HIP_OK(hipMemcpy(lb.fbits.data(), st.fbits.p, sizeof(uint64_t) * lb.fbits.size(),
+        HIP_OK(hipMemcpy(lb.fbits.data(), st.fbits.p, sizeof(uint64_t) * lb.fbits.size(),
                                hipMemcpyDeviceToHost));
     if (!lb.cbits.empty())
-        CONES_HIP_OK(hipMemcpy(lb.cbits.data(), st.cbits.p, sizeof(uint64_t) * lb.cbits.size(),
+        HIP_OK(hipMemcpy(lb.cbits.data(), st.cbits.p, sizeof(uint64_t) * lb.cbits.size(),
                                hipMemcpyDeviceToHost));
 }
 

@@ -11,19 +11,12 @@
 #include <type_traits>
 
 #include "device.hpp"
+#include "hip_util.hpp"
 
 namespace ifgf
 {
 namespace dev
 {
-
-#define HIP_OK(x)                                                                          \
-    do                                                                                     \
-    {                                                                                      \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));     \
-    } while (0)
 
 template <typename R> struct Cx
 {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "hip_util.hpp"
 #include "plan.hpp"
 #include "radix_sort.hpp"
 
@@ -26,14 +27,6 @@ namespace ifgf
 {
 namespace
 {
-#define PAIRS_HIP_OK(x)                                                                    \
-    do                                                                                     \
-    {                                                                                      \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
 struct TreeDev
 {
     const double* ctr;   // 3 per box (source tree)
@@ -119,45 +112,6 @@ struct U64Digit
     }
 };
 
-struct DBuf
-{
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() { return static_cast<T*>(p); }
-    // at least b bytes; keep the first `keep` bytes when it grows
-    void grow(size_t b, size_t keep = 0)
-    {
-        if (b <= bytes)
-            return;
-        const size_t nb = std::max<size_t>(b, bytes + bytes / 2);
-        void* q = nullptr;
-        PAIRS_HIP_OK(hipMalloc(&q, std::max<size_t>(nb, 256)));
-        if (p && keep)
-            PAIRS_HIP_OK(hipMemcpy(q, p, keep, hipMemcpyDeviceToDevice));
-        release();
-        p = q;
-        bytes = std::max<size_t>(nb, 256);
-    }
-};
-
-template <class T> void h2d(DBuf& d, const std::vector<T>& h)
-{
-    d.grow(h.size() * sizeof(T));
-    if (!h.empty())
-        PAIRS_HIP_OK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-}
-
 int bits_for(int64_t v)
 {
     int b = 0;
@@ -173,14 +127,14 @@ void sort_list(DBuf& list, DBuf& tmp, DBuf& counts, int64_t n, int key_bits,
     out.resize(n);
     if (n == 0)
         return;
-    tmp.grow(sizeof(uint64_t) * n);
-    counts.grow(sizeof(uint32_t) * radix::counts_needed(n));
+    tmp.reserve(sizeof(uint64_t) * n);
+    counts.reserve(sizeof(uint32_t) * radix::counts_needed(n));
     const int passes = (key_bits + 7) / 8;
     uint64_t* sorted = radix::sort(list.as<uint64_t>(), tmp.as<uint64_t>(), n, passes,
                                    [](int pass) { return U64Digit{8 * pass}; },
                                    counts.as<uint32_t>(), nullptr);
     std::vector<uint64_t> keys(n);
-    PAIRS_HIP_OK(hipMemcpy(keys.data(), sorted, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(keys.data(), sorted, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; i++)
         out[i] = {int32_t(keys[i] & 0xffffffffu), int32_t(keys[i] >> 32)};
 }
@@ -212,28 +166,28 @@ std::string device_build_interactions(Plan& P)
                    t_leaf.as<uint8_t>(), t_child.as<int32_t>()};
         const double eta = std::sqrt(3.0);
         DBuf gen[2], far_l, near_l, ctr;
-        ctr.grow(3 * sizeof(unsigned long long));
-        PAIRS_HIP_OK(hipMemset(ctr.p, 0, 3 * sizeof(unsigned long long)));
+        ctr.reserve(3 * sizeof(unsigned long long));
+        HIP_OK(hipMemset(ctr.p, 0, 3 * sizeof(unsigned long long)));
         const uint64_t root = 0; // (target 0, source 0)
-        gen[0].grow(sizeof(uint64_t));
-        PAIRS_HIP_OK(hipMemcpy(gen[0].p, &root, sizeof(uint64_t), hipMemcpyHostToDevice));
+        gen[0].reserve(sizeof(uint64_t));
+        HIP_OK(hipMemcpy(gen[0].p, &root, sizeof(uint64_t), hipMemcpyHostToDevice));
         int64_t n = 1, nfar = 0, nnear = 0;
         int cur = 0;
         while (n > 0)
         {
             // room for every pair of the generation in either list, 8 children each
-            far_l.grow(sizeof(uint64_t) * (nfar + n), sizeof(uint64_t) * nfar);
-            near_l.grow(sizeof(uint64_t) * (nnear + n), sizeof(uint64_t) * nnear);
-            gen[1 - cur].grow(sizeof(uint64_t) * 8 * n);
-            PAIRS_HIP_OK(hipMemset(static_cast<unsigned long long*>(ctr.p) + 2, 0,
-                                   sizeof(unsigned long long)));
+            far_l.reserve(sizeof(uint64_t) * (nfar + n), sizeof(uint64_t) * nfar);
+            near_l.reserve(sizeof(uint64_t) * (nnear + n), sizeof(uint64_t) * nnear);
+            gen[1 - cur].reserve(sizeof(uint64_t) * 8 * n);
+            HIP_OK(hipMemset(static_cast<unsigned long long*>(ctr.p) + 2, 0,
+                             sizeof(unsigned long long)));
             k_expand<<<dim3(unsigned((n + 255) / 256)), dim3(256)>>>(
                 gen[cur].as<uint64_t>(), n, SD, TD, eta, far_l.as<uint64_t>(),
                 near_l.as<uint64_t>(), gen[1 - cur].as<uint64_t>(),
                 ctr.as<unsigned long long>());
-            PAIRS_HIP_OK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             unsigned long long c[3];
-            PAIRS_HIP_OK(hipMemcpy(c, ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(c, ctr.p, sizeof(c), hipMemcpyDeviceToHost));
             nfar = int64_t(c[0]);
             nnear = int64_t(c[1]);
             n = int64_t(c[2]);

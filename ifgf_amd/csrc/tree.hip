@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "hip_util.hpp"
 #include "plan.hpp"
 #include "radix_sort.hpp"
 
@@ -35,14 +36,6 @@ namespace ifgf
 {
 namespace
 {
-#define TREE_HIP_OK(x)                                                                     \
-    do                                                                                     \
-    {                                                                                      \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
 constexpr int KEY_BITS = 40;
 constexpr int MM_THREADS = 256;
 constexpr int RS_PASSES = 16; // 8-bit digits over 128 bits (the top two are zero)
@@ -399,38 +392,6 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split(
         split[q] = 1;
 }
 
-struct DBuf
-{
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() { return static_cast<T*>(p); }
-    void reserve(size_t b)
-    {
-        if (b <= bytes)
-            return;
-        release();
-        TREE_HIP_OK(hipMalloc(&p, std::max<size_t>(b, 256)));
-        bytes = std::max<size_t>(b, 256);
-    }
-};
-
-template <class T> void h2d(DBuf& d, const T* h, size_t count)
-{
-    d.reserve(count * sizeof(T));
-    TREE_HIP_OK(hipMemcpy(d.p, h, count * sizeof(T), hipMemcpyHostToDevice));
-}
-
 int blocks_for(int64_t n, int per) { return int((n + per - 1) / per); }
 
 } // namespace
@@ -452,11 +413,11 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
         pi.reserve(sizeof(int64_t) * 6 * mm_blocks);
         k_minmax<<<mm_blocks, MM_THREADS>>>(d_pts.as<double>(), n, pv.as<double>(),
                                             pi.as<int64_t>());
-        TREE_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         std::vector<double> hv(6 * mm_blocks);
         std::vector<int64_t> hidx(6 * mm_blocks);
-        TREE_HIP_OK(hipMemcpy(hv.data(), pv.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
-        TREE_HIP_OK(hipMemcpy(hidx.data(), pi.p, hidx.size() * sizeof(int64_t),
+        HIP_OK(hipMemcpy(hv.data(), pv.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(hidx.data(), pi.p, hidx.size() * sizeof(int64_t),
                               hipMemcpyDeviceToHost));
         double mn[3], mx[3];
         for (int c = 0; c < 6; c++)
@@ -488,33 +449,33 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
         for (int b2 = 0; b2 < 2; b2++)
             keys[b2].reserve(sizeof(KeyRec) * n);
         k_keys<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), n, nm, keys[0].as<KeyRec>());
-        TREE_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         counts.reserve(sizeof(uint32_t) * radix::counts_needed(n));
         KeyRec* sorted = radix::sort(
             keys[0].as<KeyRec>(), keys[1].as<KeyRec>(), n, RS_PASSES,
             [](int pass) { return KeyDigit{pass >= 8 ? 1 : 0, (pass & 7) * 8}; },
             counts.as<uint32_t>(), nullptr);
         flag.reserve(sizeof(int));
-        TREE_HIP_OK(hipMemset(flag.p, 0, sizeof(int)));
+        HIP_OK(hipMemset(flag.p, 0, sizeof(int)));
         {
             // the indices index the points below: they must be a permutation
             DBuf seen;
             seen.reserve(sizeof(uint32_t) * n);
-            TREE_HIP_OK(hipMemset(seen.p, 0, sizeof(uint32_t) * n));
+            HIP_OK(hipMemset(seen.p, 0, sizeof(uint32_t) * n));
             k_check_perm<<<blocks_for(n, 256), 256>>>(sorted, n, seen.as<uint32_t>(),
                                                       flag.as<int>());
-            TREE_HIP_OK(hipGetLastError());
+            HIP_OK(hipGetLastError());
             int bad = 0;
-            TREE_HIP_OK(hipMemcpy(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost));
             if (bad)
                 return "device tree: the radix sort did not produce a permutation";
         }
         k_ties<<<blocks_for(n, 256), 256>>>(sorted, n, d_pts.as<double>(), nm, flag.as<int>());
-        TREE_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         int long_run = 0;
-        TREE_HIP_OK(hipMemcpy(&long_run, flag.p, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&long_run, flag.p, sizeof(int), hipMemcpyDeviceToHost));
         std::vector<KeyRec> hk(n);
-        TREE_HIP_OK(hipMemcpy(hk.data(), sorted, sizeof(KeyRec) * n, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(hk.data(), sorted, sizeof(KeyRec) * n, hipMemcpyDeviceToHost));
         if (long_run)
         {
             // runs of more than TIE_RUN_MAX equal keys (coincident points): the host's stable
@@ -534,12 +495,12 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
                                      });
                 i = j;
             }
-            TREE_HIP_OK(hipMemcpy(sorted, hk.data(), sizeof(KeyRec) * n, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(sorted, hk.data(), sizeof(KeyRec) * n, hipMemcpyHostToDevice));
         }
         d_spts.reserve(sizeof(double) * 3 * n);
         k_gather_pts<<<blocks_for(n, 256), 256>>>(d_pts.as<double>(), sorted, n,
                                                   d_spts.as<double>());
-        TREE_HIP_OK(hipGetLastError());
+        HIP_OK(hipGetLastError());
         T.perm.resize(n);
         for (int64_t i = 0; i < n; i++)
             T.perm[i] = hk[i].id;
@@ -548,7 +509,7 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
         for (int b2 = 0; b2 < 2; b2++)
             keys[b2].release();
         T.pts.resize(3 * size_t(n));
-        TREE_HIP_OK(hipMemcpy(T.pts.data(), d_spts.p, sizeof(double) * 3 * n,
+        HIP_OK(hipMemcpy(T.pts.data(), d_spts.p, sizeof(double) * 3 * n,
                               hipMemcpyDeviceToHost));
 
         // levels
@@ -569,10 +530,10 @@ std::string device_build_tree(const double* pts, int64_t n, int64_t max_leaf, Tr
                           d_spts.as<double>(), l_beg.as<int64_t>(), l_end.as<int64_t>(),
                           l_lo.as<double>(), l_hi.as<double>(), leaf, l_cr.as<int64_t>(),
                           l_split.as<uint8_t>());
-                      TREE_HIP_OK(hipGetLastError());
-                      TREE_HIP_OK(hipMemcpy(cr.data(), l_cr.p, sizeof(ChildRange) * nl,
+                      HIP_OK(hipGetLastError());
+                      HIP_OK(hipMemcpy(cr.data(), l_cr.p, sizeof(ChildRange) * nl,
                                             hipMemcpyDeviceToHost));
-                      TREE_HIP_OK(hipMemcpy(split.data(), l_split.p, size_t(nl),
+                      HIP_OK(hipMemcpy(split.data(), l_split.p, size_t(nl),
                                             hipMemcpyDeviceToHost));
                   });
     }
