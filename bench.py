@@ -88,10 +88,10 @@ def time_mults(op, w_dev, steps, warmup):
     return start.elapsed_time(end) / steps, u
 
 
-def cpu_baseline(x, w, kappa, order, leaf, stride, nthreads):
+def cpu_baseline(x, w, kappa, order, leaf, tol, stride, nthreads):
     """FP64 CPU restatement of the reference (the oracle) on every stride-th level-2 subtree."""
     from oracle import pyoracle as po
-    orc = po.Oracle(po.HELMHOLTZ, kappa, leaf, order, 1, -1)
+    orc = po.Oracle(po.HELMHOLTZ, kappa, leaf, order, 1, tol)
     t0 = time.time()
     orc.init(x, x, nthreads=nthreads)
     t_init = time.time() - t0
@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--kappa", type=float, default=100.0)
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=-1.0,
+                    help="IFGF tolerance: > 0 lets the order estimate raise the orders "
+                         "(ifgfoperator.hpp:108-244); <= 0 keeps the fixed order")
     ap.add_argument("--samples", type=int, default=1000)
     ap.add_argument("--cpu-stride", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -156,12 +159,14 @@ def main():
           flush=True)
 
     t0 = time.time()
-    op = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
+    op = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, args.tol,
                                         precision="mixed")
     op.init(x.T, x.T)
     t_init = time.time() - t0
     work = op.plan_work()
-    print(f"# init {t_init:.1f}s  {op.plan_counts()}", file=sys.stderr, flush=True)
+    orders = {k: v.tolist() for k, v in op.plan_orders().items()}
+    print(f"# init {t_init:.1f}s  {op.plan_counts()}  orders {orders}", file=sys.stderr,
+          flush=True)
     # launch shapes: timed on this operator once after init (untimed, like init)
     t0 = time.time()
     if args.launch_shape:
@@ -203,7 +208,7 @@ def main():
 
     fp64 = None
     if not args.no_fp64:
-        op64 = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
+        op64 = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, args.tol,
                                               precision="fp64")
         op64.init(x.T, x.T)
         if not (args.no_autotune or args.launch_shape):
@@ -221,7 +226,8 @@ def main():
         torch.cuda.empty_cache()
 
     fl = flops_per_mult(work)
-    default_workload = (N, args.kappa, args.order, args.leaf) == (1048576, 100.0, 6, 64)
+    default_workload = (N, args.kappa, args.order, args.leaf, args.tol) == (1048576, 100.0, 6,
+                                                                           64, -1.0)
     counters = load_counters() if default_workload else None
     names = {"near": "near", "leaf": "leaf", "upward": "upward", "transform": "transform",
              "far": "far"}
@@ -256,7 +262,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         nthreads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(x, w, args.kappa, args.order, args.leaf, args.cpu_stride, nthreads)
+        cpu = cpu_baseline(x, w, args.kappa, args.order, args.leaf, args.tol, args.cpu_stride,
+                           nthreads)
 
     line = {
         "metric": "IFGF matvecs/sec at N=1M, kappa=100 (Helmholtz single layer, mixed FP32 form)",
@@ -277,6 +284,8 @@ def main():
             "kappa": args.kappa,
             "order": args.order,
             "leaf_size": args.leaf,
+            "tolerance": args.tol,
+            "orders": orders,
             "targets": "= sources",
             "densities": "complex N(0,1), seed 2002; points seed 1002",
             "parallelism": "single GPU",

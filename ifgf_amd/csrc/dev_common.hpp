@@ -853,16 +853,23 @@ inline KParams kparams(const DevicePlan& D)
     return k;
 }
 
-// which compile-time order set matches the plan: 1 = (3,6,6)/(2,3,3)/3, 2 = (5,8,8)/(2,5,5)/3
+// which compile-time order set (high / low orders, fine factors 3) matches the plan:
+// 1 = (3,6,6)/(2,3,3), order 6 without a tolerance; 2 = (5,8,8)/(2,5,5), order 8 without a
+// tolerance; 3 = (6,8,7)/(3,5,4), what the tolerance estimate picks for order 6 at tol 1e-5 on
+// the unit sphere at N = 1M, kappa = 100.  0: the runtime-order kernels.
 inline int order_class(const DevicePlan& D)
 {
     const bool f3 = D.f[0] == 3 && D.f[1] == 3 && D.f[2] == 3;
-    if (f3 && D.ho[0] == 3 && D.ho[1] == 6 && D.ho[2] == 6 && D.lo[0] == 2 && D.lo[1] == 3 &&
-        D.lo[2] == 3)
+    auto is = [&](int h0, int h1, int h2, int l0, int l1, int l2) {
+        return f3 && D.ho[0] == h0 && D.ho[1] == h1 && D.ho[2] == h2 && D.lo[0] == l0 &&
+               D.lo[1] == l1 && D.lo[2] == l2;
+    };
+    if (is(3, 6, 6, 2, 3, 3))
         return 1;
-    if (f3 && D.ho[0] == 5 && D.ho[1] == 8 && D.ho[2] == 8 && D.lo[0] == 2 && D.lo[1] == 5 &&
-        D.lo[2] == 5)
+    if (is(5, 8, 8, 2, 5, 5))
         return 2;
+    if (is(6, 8, 7, 3, 5, 4))
+        return 3;
     return 0;
 }
 

@@ -793,18 +793,21 @@ void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeT
         return launch_up_far_t<PH, R, 2, 3, 3>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 2:
         return launch_up_far_t<PH, R, 2, 5, 5>(D, st, kp, tabs, up, far, a, b, F, V, res);
+    case 3:
+        return launch_up_far_t<PH, R, 3, 5, 4>(D, st, kp, tabs, up, far, a, b, F, V, res);
     default:
         return launch_up_far_t<PH, R, 0, 0, 0>(D, st, kp, tabs, up, far, a, b, F, V, res);
     }
 }
 
 // transform cones per workgroup the compiled kernels offer per (order class, real size), the
-// default first (LDS: 6.5 KB per cone at order 6 in FP32, 16.5 KB at order 8)
+// default first (LDS: 6.5 KB per cone at order 6 in FP32, 16.5 KB at order 8, 20 KB for
+// (6,8,7)/(3,5,4); twice that in FP64)
 static std::vector<int> transform_cpb_choices(int cls, size_t rsize)
 {
     if (cls == 1)
         return rsize == 4 ? std::vector<int>{4, 2, 6, 8} : std::vector<int>{3, 1, 2, 4};
-    if (cls == 2)
+    if (cls == 2 || cls == 3)
         return rsize == 4 ? std::vector<int>{2, 1, 3} : std::vector<int>{1};
     return {0};
 }
@@ -871,6 +874,29 @@ void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t 
             [[fallthrough]];
         default:
             IFGF_TRANSFORM(5, 8, 8, 2, 5, 5, 1);
+        }
+        return;
+    }
+    if (cls == 3)
+    {
+        switch (cpb_sel)
+        {
+        case 3:
+            if constexpr (sizeof(R) == 4)
+            {
+                IFGF_TRANSFORM(6, 8, 7, 3, 5, 4, 3);
+                break;
+            }
+            [[fallthrough]];
+        case 2:
+            if constexpr (sizeof(R) == 4)
+            {
+                IFGF_TRANSFORM(6, 8, 7, 3, 5, 4, 2);
+                break;
+            }
+            [[fallthrough]];
+        default:
+            IFGF_TRANSFORM(6, 8, 7, 3, 5, 4, 1);
         }
         return;
     }

@@ -70,6 +70,19 @@ def test_sphere_matches_oracle(gpu, prec, kind, k, leaf, order, n_elem, tol, N):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tolerance_orders_class_matches_oracle(gpu, prec):
+    """The compiled kernels of the orders (6,8,7)/(3,5,4), which the tolerance estimate
+    (ifgfoperator.hpp:108-244) picks for order 6 at tol 1e-5 here, against the oracle."""
+    N, k = 12000, 16.0
+    x = sphere(N, 100 + N % 97)
+    w = cdens(N, 200)
+    u, ref, _, op = run_pair(po.HELMHOLTZ, k, 64, 6, 1, 1e-5, x, x, w, prec)
+    o = op.plan_orders()
+    assert o["high"].tolist() == [6, 8, 7] and o["low"].tolist() == [3, 5, 4], o
+    assert rel(u, ref) < TOL[prec], rel(u, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_normals_kernels_match_oracle(gpu, prec):
     x = spheroid(7000, 31)
     y = sphere(5000, 32) * 1.3
