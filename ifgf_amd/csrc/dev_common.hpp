@@ -500,6 +500,52 @@ __device__ Cx<R> clen3r(const Cx<R>* c, int o0, int o1, int o2, R x0, R x1, R x2
     return {x2v * b1.x - b2.x + c0.x, x2v * b1.y - b2.y + c0.y};
 }
 
+// the value of clen3c on the coefficients of fine slot `slot`, loaded a chunk of axis-2 planes
+// at a time (chunks start on 16-byte boundaries), so that fewer coefficients are live at once;
+// the same operations in the same order as clen3c
+template <int O0, int O1, int O2, typename R>
+__device__ __forceinline__ Cx<R> fine_eval_chunked(const Cx<R>* __restrict__ F, int slot, R x0,
+                                                   R x1, R x2)
+{
+    constexpr int NP = O0 * O1;
+    constexpr int PC = (sizeof(R) == 4 && (NP % 2) != 0) ? 2 : 1; // planes per chunk
+    static_assert(O2 % PC == 0, "whole chunks of planes");
+    const Cx<R>* p = F + int64_t(slot) * (NP * O2);
+    Cx<R> plane[O2];
+#pragma unroll
+    for (int i2 = 0; i2 < O2; i2 += PC)
+    {
+        Cx<R> c[NP * PC];
+        if constexpr (sizeof(R) == 4 && (NP * PC) % 2 == 0)
+        {
+            const float4* p4 = reinterpret_cast<const float4*>(p + NP * i2);
+#pragma unroll
+            for (int i = 0; i < NP * PC / 2; i++)
+            {
+                const float4 v = p4[i];
+                c[2 * i] = {v.x, v.y};
+                c[2 * i + 1] = {v.z, v.w};
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < NP * PC; i++)
+                c[i] = p[NP * i2 + i];
+        }
+#pragma unroll
+        for (int k = 0; k < PC; k++)
+        {
+            Cx<R> line[O1];
+#pragma unroll
+            for (int i1 = 0; i1 < O1; i1++)
+                line[i1] = clen1c<O0, R>(c + NP * k + O0 * i1, x0);
+            plane[i2 + k] = clen1c<O1, R>(line, x1);
+        }
+    }
+    return clen1c<O2, R>(plane, x2);
+}
+
 // value of the fine interpolant in slot `slot` at local coordinates xi
 template <int O0, int O1, int O2, typename R>
 __device__ __forceinline__ Cx<R> fine_eval(const Cx<R>* __restrict__ F, int slot, const KParams& kp,

@@ -332,7 +332,11 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
                 if (!((cr.mask >> E.sub) & 1u))
                     continue;
                 const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
-                const Cx<R> val = fine_eval<O0, O1, O2, R>(F, slot, kp, E.xi0, E.xi1, E.xi2);
+                Cx<R> val;
+                if constexpr (O0 > 0)
+                    val = fine_eval_chunked<O0, O1, O2, R>(F, slot, E.xi0, E.xi1, E.xi2);
+                else
+                    val = fine_eval<O0, O1, O2, R>(F, slot, kp, E.xi0, E.xi1, E.xi2);
                 acc[k] = cadd(acc[k], cmul(val, Cx<R>{E.tr, E.ti}));
             }
         }
