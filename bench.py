@@ -2,10 +2,13 @@
 
 Workload (the headline configuration): N = 1,048,576 points uniformly random on the unit
 sphere, targets = sources, kappa = 100, Chebyshev order 6, leaf size 64, complex N(0,1)
-densities.  A step is one mult (one matvec) with the densities already resident in HBM.
-The mixed form (FP32 arithmetic, FP64 cone coordinates / phases / geometry) is the measured
-value; the plain FP64 form is timed beside it, and the relative error of both against an
-FP64 direct sum on sampled targets is reported.
+densities, at relative error <= 1e-4 against the direct sum: the IFGF tolerance 1e-5 lets the
+order estimate (ifgfoperator.hpp:108-244) raise the orders to (6, 8, 7), which gives 6.4e-5;
+order 6 without a tolerance gives 1.35e-3 and is timed beside it (`fixed_order`).  A step is
+one mult (one matvec) with the densities already resident in HBM.  The mixed form (FP32
+arithmetic, FP64 cone coordinates / phases / geometry) is the measured value; the plain FP64
+form is timed beside it, and the relative error of both against an FP64 direct sum on
+sampled targets is reported.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
 """
@@ -48,16 +51,21 @@ def direct_sum_torch(x, w, idx, kappa, dev):
 
 KERNEL_OF_PHASE = {"near": "k_near", "leaf": "k_leaf", "upward": "k_up_cell",
                    "transform": "k_transform_t", "far": "k_far"}
-COUNTERS = os.path.join(HERE, "profiles", "counters_helmholtz_sphere_1M_kappa100_order6_mixed.json")
+# rocprofv3 counter summaries of the 1M-point workloads, by tolerance
+COUNTERS = {
+    1e-5: os.path.join(HERE, "profiles",
+                       "counters_helmholtz_sphere_1M_kappa100_order6_tol1e-5_mixed.json"),
+    -1.0: os.path.join(HERE, "profiles", "counters_helmholtz_sphere_1M_kappa100_order6_mixed.json"),
+}
 
 
-def load_counters():
-    """HBM bytes per mult and launches per mult of each kernel at the default workload, from
+def load_counters(tol):
+    """HBM bytes per mult and launches per mult of each kernel at a 1M-point workload, from
     the committed rocprofv3 counter summary (None when absent)."""
     try:
-        with open(COUNTERS) as f:
+        with open(COUNTERS[tol]) as f:
             return json.load(f)
-    except (OSError, ValueError):
+    except (KeyError, OSError, ValueError):
         return None
 
 
@@ -119,11 +127,13 @@ def main():
     ap.add_argument("--kappa", type=float, default=100.0)
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--leaf", type=int, default=64)
-    ap.add_argument("--tol", type=float, default=-1.0,
+    ap.add_argument("--tol", type=float, default=1e-5,
                     help="IFGF tolerance: > 0 lets the order estimate raise the orders "
                          "(ifgfoperator.hpp:108-244); <= 0 keeps the fixed order")
+    ap.add_argument("--no-fixed-order", action="store_true",
+                    help="skip the run of the same order without a tolerance")
     ap.add_argument("--samples", type=int, default=1000)
-    ap.add_argument("--cpu-stride", type=int, default=2)
+    ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -225,10 +235,36 @@ def main():
         del op64
         torch.cuda.empty_cache()
 
+    # the same order without a tolerance (orders (3, 6, 6) / (2, 3, 3): faster, less accurate)
+    fixed = None
+    if not args.no_fixed_order and args.tol > 0:
+        del op
+        torch.cuda.empty_cache()
+        opf = ifgf_amd.HelmholtzIfgfOperator(args.kappa, args.leaf, args.order, 1, -1,
+                                             precision="mixed")
+        t0 = time.time()
+        opf.init(x.T, x.T)
+        tf_init = time.time() - t0
+        shape_f = (opf.launch_shape if (args.no_autotune or args.launch_shape)
+                   else opf.autotune(nrhs=1, reps=3))
+        msf, uf = time_mults(opf, w_dev, args.steps, args.warmup)
+        uf_np = uf.cpu().numpy()
+        fixed = {
+            "tolerance": -1.0,
+            "orders": {k: v.tolist() for k, v in opf.plan_orders().items()},
+            "ms_per_step": msf,
+            "value": 1000.0 / msf,
+            "rel_err_vs_direct": float(np.linalg.norm(uf_np[idx] - ref) / np.linalg.norm(ref)),
+            "init_s": tf_init,
+            "launch_shape": shape_f,
+        }
+        print(f"# fixed order: {msf:.3f} ms/matvec", file=sys.stderr, flush=True)
+        del opf
+        torch.cuda.empty_cache()
+
     fl = flops_per_mult(work)
-    default_workload = (N, args.kappa, args.order, args.leaf, args.tol) == (1048576, 100.0, 6,
-                                                                           64, -1.0)
-    counters = load_counters() if default_workload else None
+    full_size = (N, args.kappa, args.order, args.leaf) == (1048576, 100.0, 6, 64)
+    counters = load_counters(args.tol) if full_size else None
     names = {"near": "near", "leaf": "leaf", "upward": "upward", "transform": "transform",
              "far": "far"}
     dom = max(names, key=lambda k: phases.get(k, 0.0))
@@ -266,7 +302,8 @@ def main():
                            nthreads)
 
     line = {
-        "metric": "IFGF matvecs/sec at N=1M, kappa=100 (Helmholtz single layer, mixed FP32 form)",
+        "metric": "IFGF matvecs/sec at N=1M, kappa=100, rel.err<=1e-4 vs FP64 direct; "
+                  "FP32-mixed vs plain FP64",
         "value": 1000.0 / ms,
         "unit": "matvecs/s",
         "n_gpus": 1,
@@ -297,6 +334,7 @@ def main():
         "launch_shape": shape,
         "autotune_s": t_tune,
         "fp64_form": fp64,
+        "fixed_order": fixed,
         "mixed_over_fp64": (fp64["ms_per_step"] / ms) if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
