@@ -49,8 +49,10 @@ def direct_sum_torch(x, w, idx, kappa, dev):
     return torch.cat(out).cpu().numpy()
 
 
-KERNEL_OF_PHASE = {"near": "k_near", "leaf": "k_leaf", "upward": "k_up_cell",
-                   "transform": "k_transform_t", "far": "k_far"}
+# the kernels of each phase, the first one found in a counter summary is the phase's
+KERNEL_OF_PHASE = {"near": ("k_near",), "leaf": ("k_leaf_box", "k_leaf"),
+                   "upward": ("k_up_lds", "k_up_cell"), "transform": ("k_transform_t",),
+                   "far": ("k_far",)}
 # rocprofv3 counter summaries of the 1M-point workloads, by tolerance
 COUNTERS = {
     1e-5: os.path.join(HERE, "profiles",
@@ -271,7 +273,10 @@ def main():
     t_dom = phases[dom] * 1e-3
     achieved = fl[dom] / t_dom / 1e12 if t_dom > 0 else 0.0
     traffic = None
-    kc = counters.get("kernels", {}).get(KERNEL_OF_PHASE[dom]) if counters else None
+    kc = None
+    if counters:
+        found = [k for k in KERNEL_OF_PHASE[dom] if k in counters.get("kernels", {})]
+        kc = counters["kernels"][found[0]] if found else None
     if kc:
         traffic = (kc["fetch_bytes_per_mult"] + kc["write_bytes_per_mult"]) / kc["launches_per_mult"]
     roofline = {

@@ -859,7 +859,8 @@ __device__ __forceinline__ int pow2_at_least(int n)
 struct UpEntF
 {
     int32_t cf, sub;
-    float xi0, xi1, xi2, tr, ti, pad;
+    float xi0, xi1, xi2, tr, ti;
+    int32_t d; // index of (cf, sub) among the distinct child cones of its (cell, octant)
 };
 struct UpEntD
 {

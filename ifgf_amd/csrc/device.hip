@@ -347,6 +347,130 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
     }
 }
 
+// upward pass with the per-level tables and the child coefficients staged in LDS (FP32,
+// compiled orders).  The entries of one (cell, octant) fall into few distinct child cones
+// (about 15 for 336 nodes at orders (6,8,7)/(3,5,4)), listed per (cell, octant) at upload
+// (UpEntF::d).  Per parent cone k of the item and octant c the workgroup copies the
+// coefficient blocks of those cones into LDS once, and every node reads its block from there
+// (k_up_cell loads a block from memory per node, and the loads bound it).  Lists longer than
+// DMAX read the rest from memory.  Per node the same evaluations and the same sum over the
+// children in the same order as k_up_cell.
+template <int O0, int O1, int O2, int K, int QPT, int DMAX>
+__global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ items,
+                                                int64_t nitems,
+                                                const int32_t* __restrict__ glist,
+                                                const ConeDev* __restrict__ cones,
+                                                const BoxGeo* __restrict__ geo,
+                                                const int32_t* __restrict__ child,
+                                                const CellRef* __restrict__ ctab,
+                                                const UpEntF* __restrict__ ents,
+                                                const int64_t* __restrict__ part_ent,
+                                                const int64_t* __restrict__ part_cell,
+                                                const int32_t* __restrict__ doff,
+                                                const int32_t* __restrict__ dkey,
+                                                const Cx<float>* __restrict__ F,
+                                                Cx<float>* __restrict__ V, KParams kp)
+{
+    constexpr int NC = O0 * O1 * O2;
+    static_assert(NC % 2 == 0, "blocks of whole float4");
+    constexpr int NV = NC / 2;
+    __shared__ int64_t toff[K][8]; // table offset of child c of cone k; -1: no child
+    __shared__ int32_t gs[K];
+    __shared__ float4 blk[DMAX * NV];
+    __shared__ int32_t bslot[DMAX]; // slot of staged cone d; -1: not active in the child
+    const int64_t it = xcd_block();
+    if (it >= nitems)
+        return;
+    const WorkItem wi = items[it];
+    const int tid = threadIdx.x;
+    if (tid < K * 8)
+    {
+        const int k = tid >> 3, c = tid & 7;
+        int64_t off = -1;
+        if (k < wi.count)
+        {
+            const int32_t g = glist[wi.first + k];
+            if (c == 0)
+                gs[k] = g;
+            const int32_t ch = child[8 * cones[g].box + c];
+            if (ch >= 0)
+                off = geo[ch].table_off;
+        }
+        toff[k][c] = off;
+    }
+    __syncthreads();
+    const int n = wi.count;
+    const int nhi = kp.n_hi;
+    const UpEntF* ebox = ents + part_ent[wi.pad] + int64_t(wi.box) * nhi * 8;
+    const int64_t row = (part_cell[wi.pad] + wi.box) * 8;
+    const float4* F4 = reinterpret_cast<const float4*>(F);
+#pragma unroll 1
+    for (int k = 0; k < n; k++)
+    {
+        Cx<float> acc[QPT];
+#pragma unroll
+        for (int i = 0; i < QPT; i++)
+            acc[i] = {0, 0};
+#pragma unroll 1
+        for (int c = 0; c < 8; c++)
+        {
+            const int64_t to = toff[k][c];
+            if (to < 0) // the same for the whole workgroup
+                continue;
+            const int d0 = doff[row + c];
+            const int nd = min(doff[row + c + 1] - d0, DMAX);
+            for (int i = tid; i < nd * NV; i += 128)
+            {
+                const int d = i / NV, j = i - d * NV;
+                const int32_t cf = dkey[2 * (d0 + d)], sub = dkey[2 * (d0 + d) + 1];
+                const CellRef cr = ctab[to + cf];
+                const bool on = (cr.mask >> sub) & 1u;
+                const int slot = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
+                if (on)
+                    blk[i] = F4[int64_t(slot) * NV + j];
+                if (j == 0)
+                    bslot[d] = on ? slot : -1;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < QPT; i++)
+            {
+                const int q = tid + 128 * i;
+                if (q >= nhi)
+                    continue;
+                const UpEntF E = ebox[int64_t(q) * 8 + c];
+                if (E.cf < 0)
+                    continue;
+                Cx<float> val;
+                if (E.d < DMAX)
+                {
+                    if (bslot[E.d] < 0)
+                        continue;
+                    val = fine_eval_chunked<O0, O1, O2, float>(
+                        reinterpret_cast<const Cx<float>*>(blk), E.d, E.xi0, E.xi1, E.xi2);
+                }
+                else
+                {
+                    const CellRef cr = ctab[to + E.cf];
+                    if (!((cr.mask >> E.sub) & 1u))
+                        continue;
+                    const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
+                    val = fine_eval_chunked<O0, O1, O2, float>(F, slot, E.xi0, E.xi1, E.xi2);
+                }
+                acc[i] = cadd(acc[i], cmul(val, Cx<float>{E.tr, E.ti}));
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < QPT; i++)
+        {
+            const int q = tid + 128 * i;
+            if (q < nhi)
+                V[int64_t(gs[k]) * nhi + q] = acc[i];
+        }
+    }
+}
+
 // fused high-order values -> fine coefficients of the active sub-cones, separable per axis:
 // D = (W2 x W1 x W0) V with W_d the (f o_lo) x o_hi axis maps (chebinterp.hpp:176-250,
 // ifgfoperator.hpp:1539-1637 and the fine transform :548-570).  One thread per pencil of an
@@ -720,7 +844,7 @@ bool g_force_timing = false; // phase timing on (the autotuner reads single phas
 
 // up_level >= 0: the upward pass into parent level up_level; otherwise the far field of
 // target level far_level
-template <bool PH, typename R, int O0, int O1, int O2>
+template <bool PH, typename R, int H0, int H1, int H2, int O0, int O1, int O2>
 void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs,
                      int up_level, int far_level, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V,
                      Cx<R>* res)
@@ -732,9 +856,17 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
         if (i1 > i0)
         {
 #define IFGF_UP_CELL(K)                                                                         \
-    k_up_cell<R, O0, O1, O2, K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                \
-        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo, D.child,  \
-        D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp)
+    if constexpr (sizeof(R) == 4 && O0 * O1 * O2 > 18)                                          \
+        k_up_lds<O0, O1, O2, K, (H0 * H1 * H2 + 127) / 128, 32>                                 \
+            <<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                                    \
+                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,   \
+                D.child, D.ctab, static_cast<const UpEntF*>(D.up_entries), D.up_part_ent,       \
+                D.up_part_cell, D.up_doff, D.up_dkey, F, V, kp);                                \
+    else                                                                                        \
+        k_up_cell<R, O0, O1, O2, K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(            \
+            D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,       \
+            D.child, D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V,  \
+            kp)
             switch (D.shape.up_k)
             {
             case 4:
@@ -794,13 +926,13 @@ void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeT
     switch (order_class(D))
     {
     case 1:
-        return launch_up_far_t<PH, R, 2, 3, 3>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 3, 6, 6, 2, 3, 3>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 2:
-        return launch_up_far_t<PH, R, 2, 5, 5>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 5, 8, 8, 2, 5, 5>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 3:
-        return launch_up_far_t<PH, R, 3, 5, 4>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 6, 8, 7, 3, 5, 4>(D, st, kp, tabs, up, far, a, b, F, V, res);
     default:
-        return launch_up_far_t<PH, R, 0, 0, 0>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 0, 0, 0, 0, 0, 0>(D, st, kp, tabs, up, far, a, b, F, V, res);
     }
 }
 
@@ -1121,7 +1253,8 @@ void device_free(DevicePlan& D)
 {
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
-                    D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_items_k[0],
+                    D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_dkey,
+                    D.up_doff, D.up_part_cell, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_glist,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
@@ -1350,21 +1483,66 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         {
             std::vector<UpEntF> ef;
             std::vector<UpEntD> ed;
-            std::vector<int64_t> part_ent;
+            std::vector<int64_t> part_ent, part_cell;
+            std::vector<int32_t> dkey, doff;
             for (const UpLevel* U : parts)
             {
                 part_ent.push_back(int64_t(precision == PREC_FP64 ? ed.size() : ef.size()));
-                for (const UpEntry& E : U->entries)
+                part_cell.push_back(int64_t(doff.size() / 8));
+                if (precision == PREC_FP64)
                 {
-                    if (precision == PREC_FP64)
+                    for (const UpEntry& E : U->entries)
                         ed.push_back({E.cf, E.sub, E.xi[0], E.xi[1], E.xi[2], E.tf[0], E.tf[1]});
-                    else
-                        ef.push_back({E.cf, E.sub, float(E.xi[0]), float(E.xi[1]), float(E.xi[2]),
-                                      float(E.tf[0]), float(E.tf[1]), 0.f});
+                    continue;
                 }
+                const size_t e0 = ef.size();
+                for (const UpEntry& E : U->entries)
+                    ef.push_back({E.cf, E.sub, float(E.xi[0]), float(E.xi[1]), float(E.xi[2]),
+                                  float(E.tf[0]), float(E.tf[1]), -1});
+                // the distinct child cones per (cell, octant); neighbouring nodes mostly share
+                // the last one found
+                const int64_t ncell = int64_t(U->entries.size()) / (int64_t(D.n_hi) * 8);
+                for (int64_t cell = 0; cell < ncell; cell++)
+                    for (int c = 0; c < 8; c++)
+                    {
+                        const int32_t k0 = int32_t(dkey.size() / 2);
+                        doff.push_back(k0);
+                        int last = -1;
+                        for (int q = 0; q < D.n_hi; q++)
+                        {
+                            UpEntF& E = ef[e0 + size_t((cell * D.n_hi + q) * 8 + c)];
+                            if (E.cf < 0)
+                                continue;
+                            const int nk = int(dkey.size() / 2) - k0;
+                            auto is = [&](int i) {
+                                return dkey[2 * (k0 + i)] == E.cf && dkey[2 * (k0 + i) + 1] == E.sub;
+                            };
+                            int d = (last >= 0 && is(last)) ? last : -1;
+                            for (int i = 0; d < 0 && i < nk; i++)
+                                if (is(i))
+                                    d = i;
+                            if (d < 0)
+                            {
+                                d = nk;
+                                dkey.push_back(E.cf);
+                                dkey.push_back(E.sub);
+                            }
+                            E.d = d;
+                            last = d;
+                        }
+                    }
             }
+            if (dkey.size() / 2 > size_t(INT32_MAX))
+                throw std::runtime_error("upward tables: too many distinct child cones");
+            doff.push_back(int32_t(dkey.size() / 2));
             D.n_up_parts = int64_t(parts.size());
             D.up_part_ent = dupload(part_ent, tot);
+            if (precision != PREC_FP64)
+            {
+                D.up_part_cell = dupload(part_cell, tot);
+                D.up_doff = dupload(doff, tot);
+                D.up_dkey = dupload(dkey, tot);
+            }
             if (precision == PREC_FP64)
                 D.up_entries = dupload(ed, tot);
             else

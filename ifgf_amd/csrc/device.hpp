@@ -110,6 +110,12 @@ struct DevicePlan
     void* up_entries = nullptr;
     int64_t* up_part_ent = nullptr;
     int64_t n_up_parts = 0;
+    // FP32 tables: per (cell, octant) the distinct child cones (cf, sub) its entries fall into,
+    // in order of first appearance (UpEntF::d indexes them): up_dkey pairs, (cell, octant) row
+    // r = (up_part_cell[part] + cell) * 8 + octant spans up_dkey[2 up_doff[r] .. 2 up_doff[r+1])
+    int32_t* up_dkey = nullptr;
+    int32_t* up_doff = nullptr;
+    int64_t* up_part_cell = nullptr;
 
     // (cell, cones of that cell, part) items of at most UP_K_CHOICES[i] cones, one list per
     // choice; up_items / up_item_off name the list of the current shape

@@ -32,6 +32,25 @@ def test_every_shape_gives_the_same_result(gpu, prec, order):
         assert np.array_equal(op.mult(w), ref), (lg, cpb, k, fw, ov)
 
 
+def test_tolerance_orders_every_shape_gives_the_same_result(gpu):
+    """The orders (6,8,7)/(3,5,4) (the tolerance estimate at tol 1e-5 on this geometry): the
+    FP32 upward pass stages the child coefficients in LDS per parent cone, so the cones per
+    item change the staging order but not the sums."""
+    x = sphere(12000, 100 + 12000 % 97)
+    w = cdens(12000, 607)
+    op = ifgf_amd.HelmholtzIfgfOperator(16.0, 64, 6, 1, 1e-5)
+    op.init(x.T, x.T)
+    o = op.plan_orders()
+    assert o["high"].tolist() == [6, 8, 7] and o["low"].tolist() == [3, 5, 4], o
+    ref = op.mult(w)
+    for lg, cpb, k, fw, ov in itertools.product([0, 4], [2, 1, 3], [4, 8, 16], [1, 4], [0, 1]):
+        op.launch_shape = dict(xcd_run_log2=lg, transform_cones=cpb, up_cones=k, far_waves=fw,
+                               overlap=ov)
+        print(f"shape (6,8,7) xcd_log2={lg} cpb={cpb} up_k={k} far_waves={fw} overlap={ov}",
+              flush=True)
+        assert np.array_equal(op.mult(w), ref), (lg, cpb, k, fw, ov)
+
+
 def test_batched_shapes_and_autotune(gpu):
     import torch
     x = spheroid(20000, 603)
