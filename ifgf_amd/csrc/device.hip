@@ -377,7 +377,10 @@ __global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ ite
     __shared__ int64_t toff[K][8]; // table offset of child c of cone k; -1: no child
     __shared__ int32_t gs[K];
     __shared__ float4 blk[DMAX * NV];
-    __shared__ int32_t bslot[DMAX]; // slot of staged cone d; -1: not active in the child
+    // slot of listed cone d in child c of cone k (-1: not active in the child), found for the
+    // whole item up front so that each staging step waits for one load only
+    __shared__ int32_t sslot[K][8][DMAX];
+    __shared__ int32_t dcnt[8];
     const int64_t it = xcd_block();
     if (it >= nitems)
         return;
@@ -404,6 +407,22 @@ __global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ ite
     const UpEntF* ebox = ents + part_ent[wi.pad] + int64_t(wi.box) * nhi * 8;
     const int64_t row = (part_cell[wi.pad] + wi.box) * 8;
     const float4* F4 = reinterpret_cast<const float4*>(F);
+    if (tid < 8)
+        dcnt[tid] = min(doff[row + tid + 1] - doff[row + tid], DMAX);
+    __syncthreads();
+    for (int i = tid; i < n * 8 * DMAX; i += 128)
+    {
+        const int k = i / (8 * DMAX), c = (i / DMAX) & 7, d = i % DMAX;
+        const int64_t to = toff[k][c];
+        if (to < 0 || d >= dcnt[c])
+            continue;
+        const int kd = doff[row + c] + d;
+        const int32_t cf = dkey[2 * kd], sub = dkey[2 * kd + 1];
+        const CellRef cr = ctab[to + cf];
+        sslot[k][c][d] =
+            ((cr.mask >> sub) & 1u) ? cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u)) : -1;
+    }
+    __syncthreads();
 #pragma unroll 1
     for (int k = 0; k < n; k++)
     {
@@ -417,19 +436,13 @@ __global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ ite
             const int64_t to = toff[k][c];
             if (to < 0) // the same for the whole workgroup
                 continue;
-            const int d0 = doff[row + c];
-            const int nd = min(doff[row + c + 1] - d0, DMAX);
+            const int nd = dcnt[c];
             for (int i = tid; i < nd * NV; i += 128)
             {
                 const int d = i / NV, j = i - d * NV;
-                const int32_t cf = dkey[2 * (d0 + d)], sub = dkey[2 * (d0 + d) + 1];
-                const CellRef cr = ctab[to + cf];
-                const bool on = (cr.mask >> sub) & 1u;
-                const int slot = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
-                if (on)
+                const int slot = sslot[k][c][d];
+                if (slot >= 0)
                     blk[i] = F4[int64_t(slot) * NV + j];
-                if (j == 0)
-                    bslot[d] = on ? slot : -1;
             }
             __syncthreads();
 #pragma unroll
@@ -444,7 +457,7 @@ __global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ ite
                 Cx<float> val;
                 if (E.d < DMAX)
                 {
-                    if (bslot[E.d] < 0)
+                    if (sslot[k][c][E.d] < 0)
                         continue;
                     val = fine_eval_chunked<O0, O1, O2, float>(
                         reinterpret_cast<const Cx<float>*>(blk), E.d, E.xi0, E.xi1, E.xi2);
