@@ -900,6 +900,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V)
             switch (D.shape.up_k)
             {
+            case 2: // items of at most 2 cones run as items of 4
             case 4:
                 IFGF_UP_STAGE_B(4);
                 break;
@@ -920,6 +921,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp)
             switch (D.shape.up_k)
             {
+            case 2: // items of at most 2 cones run as items of 4
             case 4:
                 IFGF_UP_CELL_B(4);
                 break;

@@ -882,6 +882,13 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
             kp)
             switch (D.shape.up_k)
             {
+            case 2: // compiled for the LDS kernel only; the others take items of 2 as of 4
+                if constexpr (sizeof(R) == 4 && O0 * O1 * O2 > 18)
+                {
+                    IFGF_UP_CELL(2);
+                    break;
+                }
+                [[fallthrough]];
             case 4:
                 IFGF_UP_CELL(4);
                 break;
@@ -1268,7 +1275,7 @@ void device_free(DevicePlan& D)
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
                     D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_dkey,
                     D.up_doff, D.up_part_cell, D.up_items_k[0],
-                    D.up_items_k[1], D.up_items_k[2], D.up_glist,
+                    D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
@@ -1565,10 +1572,10 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         // grouped by (part, cell) into items of at most UP_K_CHOICES[i] cones of one cell (boxes in order
         // within a cell), and the other parent cones (per-box upward pass)
         std::vector<int32_t> leaf, up, glist;
-        std::vector<WorkItem> uitems[3];
+        std::vector<WorkItem> uitems[4];
         D.up_off.assign(D.levels + 1, 0);
         D.up_gl_off.assign(D.levels + 1, 0);
-        for (int kc = 0; kc < 3; kc++)
+        for (int kc = 0; kc < 4; kc++)
             D.up_item_off_k[kc].assign(D.levels + 1, 0);
         D.cone_lvl_off.assign(D.levels + 1, 0);
         for (int l = 0; l < D.levels; l++)
@@ -1578,7 +1585,7 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             D.cone_lvl_off[l + 1] = P.cone_off[b1];
             D.up_off[l] = int64_t(up.size());
             D.up_gl_off[l] = int64_t(glist.size());
-            for (int kc = 0; kc < 3; kc++)
+            for (int kc = 0; kc < 4; kc++)
                 D.up_item_off_k[kc][l] = int64_t(uitems[kc].size());
             const int64_t g0 = int64_t(glist.size());
             for (int64_t g = b0; g < b1; g++)
@@ -1588,7 +1595,7 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             auto cu = [&](int32_t c) { return parts[part(c)]->cellmap[P.cone_cell[c]]; };
             std::stable_sort(glist.begin() + g0, glist.end(), [&](int32_t a, int32_t b)
                              { return part(a) != part(b) ? part(a) < part(b) : cu(a) < cu(b); });
-            for (int kc = 0; kc < 3; kc++)
+            for (int kc = 0; kc < 4; kc++)
                 for (int64_t i = g0; i < int64_t(glist.size());)
                 {
                     const int32_t pa = part(glist[i]), u = cu(glist[i]);
@@ -1604,13 +1611,13 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         }
         D.up_off[D.levels] = int64_t(up.size());
         D.up_gl_off[D.levels] = int64_t(glist.size());
-        for (int kc = 0; kc < 3; kc++)
+        for (int kc = 0; kc < 4; kc++)
         {
             D.up_item_off_k[kc][D.levels] = int64_t(uitems[kc].size());
             D.up_items_k[kc] = dupload(uitems[kc], tot);
         }
-        D.up_items = D.up_items_k[2]; // the default shape: 16 cones per item
-        D.up_item_off = D.up_item_off_k[2];
+        D.up_items = D.up_items_k[3]; // the default shape: 16 cones per item
+        D.up_item_off = D.up_item_off_k[3];
         D.up_glist = dupload(glist, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
@@ -1861,11 +1868,11 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
     if (sh.xcd_log2 < 0 || sh.xcd_log2 > 8)
         return "launch shape: XCD run length must be 2^0 .. 2^8";
     int kc = -1;
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 4; i++)
         if (UP_K_CHOICES[i] == sh.up_k)
             kc = i;
     if (kc < 0)
-        return "launch shape: upward cones per item must be 4, 8 or 16";
+        return "launch shape: upward cones per item must be 2, 4, 8 or 16";
     if (sh.rb != 0 && sh.rb != 8 && sh.rb != 32)
         return "launch shape: batched columns per pass must be 8 or 32";
     if (sh.far_waves != 1 && sh.far_waves != 2 && sh.far_waves != 4 && sh.far_waves != 8)
@@ -1976,7 +1983,7 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
             if (ch.size() > 1)
                 pick(&LaunchShape::cpb, ch, 3);
         }
-        pick(&LaunchShape::up_k, {4, 8, 16}, batched ? -1 : 2);
+        pick(&LaunchShape::up_k, {2, 4, 8, 16}, batched ? -1 : 2);
         if (!batched && D.prec == PREC_MIXED)
             pick(&LaunchShape::far_waves, {1, 2, 4, 8}, 4);
         pick(&LaunchShape::overlap, {0, 1, 2}, -1);

@@ -65,13 +65,13 @@ struct LaunchShape
 {
     int xcd_log2 = 4; // work items dealt to the XCDs in runs of 2^xcd_log2 (0: plain order)
     int cpb = 0;      // transform: cones per workgroup (0: the order class's default)
-    int up_k = 16;    // upward: parent cones of one cell per workgroup (4, 8 or 16)
+    int up_k = 16;    // upward: parent cones of one cell per workgroup (2, 4, 8 or 16)
     int rb = 0;       // batched: columns per pass (8 or 32; 0: 8 up to 8 columns, else 32)
     int far_waves = 1; // mixed far field: waves (items) per workgroup sharing the LDS tables
     int overlap = 1;   // far field on a second stream beside the upward chain, with the near
                        // field there too (1) or first on the main stream (2); 0: one stream
 };
-constexpr int UP_K_CHOICES[3] = {4, 8, 16};
+constexpr int UP_K_CHOICES[4] = {2, 4, 8, 16};
 
 struct DevicePlan
 {
@@ -119,8 +119,8 @@ struct DevicePlan
 
     // (cell, cones of that cell, part) items of at most UP_K_CHOICES[i] cones, one list per
     // choice; up_items / up_item_off name the list of the current shape
-    WorkItem* up_items_k[3] = {nullptr, nullptr, nullptr};
-    std::vector<int64_t> up_item_off_k[3];
+    WorkItem* up_items_k[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<int64_t> up_item_off_k[4];
     WorkItem* up_items = nullptr;
     std::vector<int64_t> up_item_off;        // per level, into up_items
     int32_t* up_glist = nullptr;             // the items' cones, level-major
