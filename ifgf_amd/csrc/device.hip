@@ -499,12 +499,14 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
     constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2, NSUB = F0 * F1 * F2;
     constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
-    constexpr int YS = (M0 * H1 * H2 > M0 * M1 * M2) ? M0 * H1 * H2 : M0 * M1 * M2;
+    // the axis-2 stage writes the active sub-cones' coefficients to memory directly: Y holds
+    // the axis-0 stage only
+    constexpr int YS = M0 * H1 * H2;
     constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
     __shared__ Cx<R> X[CPB * XS];
     __shared__ Cx<R> Y[CPB * YS];
-    __shared__ unsigned char subl[CPB][32];
-    __shared__ int nsub[CPB];
+    __shared__ uint32_t cmask[CPB];
+    __shared__ int cbase[CPB];
     const int tid = threadIdx.x;
     const int64_t first = c0 + xcd_block() * CPB;
     const int ncone = int(min<int64_t>(CPB, c1 - first));
@@ -515,12 +517,8 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     }
     if (tid < ncone)
     {
-        const uint32_t mask = cones[first + tid].mask;
-        int n = 0;
-        for (int s = 0; s < NSUB; s++)
-            if ((mask >> s) & 1u)
-                subl[tid][n++] = (unsigned char)s;
-        nsub[tid] = n;
+        cmask[tid] = cones[first + tid].mask;
+        cbase[tid] = cones[first + tid].fine_base;
     }
     __syncthreads();
     // the axis maps at compile-time offsets from the kernel argument: scalar loads, the
@@ -573,10 +571,17 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
         }
     }
     __syncthreads();
-    // axis 2: pencils (j0, j1): Y[j0 + M0 (j1 + M1 j2)] = sum_i2 W2[j2][i2] X[j0 + M0 (j1 + M1 i2)]
+    // axis 2: pencils (j0, j1): coefficient (j0, j1, j2) = sum_i2 W2[j2][i2] X[j0 + M0 (j1 + M1 i2)],
+    // written to its sub-cone's block when the sub-cone is active (active blocks are stored
+    // contiguously from the cone's fine_base, in sub-cone order)
     for (int e = tid; e < ncone * M0 * M1; e += 256)
     {
         const int k = e / (M0 * M1), p = e - k * (M0 * M1);
+        const int j0 = p % M0, j1 = p / M0;
+        const int s0 = j0 / L0, s1 = j1 / L1;
+        const int m01 = (j0 - s0 * L0) + L0 * (j1 - s1 * L1);
+        const uint32_t mask = cmask[k];
+        const int base = cbase[k];
         Cx<R> in[H2];
 #pragma unroll
         for (int t = 0; t < H2; t++)
@@ -591,23 +596,13 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
                 a.x += W2[j * H2 + t] * in[t].x;
                 a.y += W2[j * H2 + t] * in[t].y;
             }
-            Y[k * YS + p + M0 * M1 * j] = a;
-        }
-    }
-    __syncthreads();
-    // write the active sub-cones' coefficient blocks contiguously
-    for (int k = 0; k < ncone; k++)
-    {
-        const int n = nsub[k];
-        const int base = cones[first + k].fine_base;
-        for (int e = tid; e < n * NLO; e += 256)
-        {
-            const int rk = e / NLO, coef = e - rk * NLO;
-            const int sub = subl[k][rk];
-            const int s0 = sub % F0, s1 = (sub / F0) % F1, s2 = sub / (F0 * F1);
-            const int m0 = coef % L0, m1 = (coef / L0) % L1, m2 = coef / (L0 * L1);
-            const int j0 = s0 * L0 + m0, j1 = s1 * L1 + m1, j2 = s2 * L2 + m2;
-            Fout[int64_t(base + rk) * NLO + coef] = Y[k * YS + j0 + M0 * (j1 + M1 * j2)];
+            const int s2 = j / L2, m2 = j - s2 * L2;
+            const int sub = s0 + F0 * (s1 + F1 * s2);
+            if ((mask >> sub) & 1u)
+            {
+                const int rk = __popc(mask & ((1u << sub) - 1u));
+                Fout[int64_t(base + rk) * NLO + m01 + L0 * L1 * m2] = a;
+            }
         }
     }
 }
