@@ -354,9 +354,11 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
 // coefficient blocks of those cones into LDS once, and every node reads its block from there
 // (k_up_cell loads a block from memory per node, and the loads bound it).  Lists longer than
 // DMAX read the rest from memory.  Per node the same evaluations and the same sum over the
-// children in the same order as k_up_cell.
+// children in the same order as k_up_cell.  Held to 5 waves per SIMD (no spills at 2 and 4
+// cones per item; DMAX = 24 keeps the LDS from limiting it further).
 template <int O0, int O1, int O2, int K, int QPT, int DMAX>
-__global__ __launch_bounds__(128) void k_up_lds(const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k_up_lds(
+                                                const WorkItem* __restrict__ items,
                                                 int64_t nitems,
                                                 const int32_t* __restrict__ glist,
                                                 const ConeDev* __restrict__ cones,
@@ -865,7 +867,7 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
         {
 #define IFGF_UP_CELL(K)                                                                         \
     if constexpr (sizeof(R) == 4 && O0 * O1 * O2 > 18)                                          \
-        k_up_lds<O0, O1, O2, K, (H0 * H1 * H2 + 127) / 128, 32>                                 \
+        k_up_lds<O0, O1, O2, K, (H0 * H1 * H2 + 127) / 128, 24>                                 \
             <<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                                    \
                 D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,   \
                 D.child, D.ctab, static_cast<const UpEntF*>(D.up_entries), D.up_part_ent,       \
