@@ -145,10 +145,11 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
             __syncthreads();
             for (int k = grp; k < n; k += ngrp)
             {
-                const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
-                        x2 = R(sp[3 * k + 2] - y2);
-                const Cx<R> g =
-                    p2p<KIND, R>(kb, x0, x1, x2, NRM ? sn + 3 * k : nullptr, Cx<R>{1, 0});
+                const double d0 = sp[3 * k] - y0, d1 = sp[3 * k + 1] - y1, d2 = sp[3 * k + 2] - y2;
+                double xn = 0;
+                if constexpr (NRM)
+                    xn = d0 * sn[3 * k] + d1 * sn[3 * k + 1] + d2 * sn[3 * k + 2];
+                const Cx<R> g = p2p<KIND, R>(kb, R(d0), R(d1), R(d2), xn, Cx<R>{1, 0});
 #pragma unroll
                 for (int r = 0; r < RB; r++)
                     acc[r] = cadd(acc[r], cmul(g, sw[k * RB + r]));

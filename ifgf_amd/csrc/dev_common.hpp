@@ -608,10 +608,12 @@ template <typename R> __device__ __forceinline__ R fdiv_r(R a, R b)
 }
 
 template <int KIND, typename R>
-__device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const double* nrm,
-                                     Cx<R> w)
+__device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, double xn, Cx<R> w)
 {
-    // (x0,x1,x2) = source - target
+    // (x0,x1,x2) = source - target; xn = n . (source - target) with the source normal n,
+    // formed by the caller in FP64 from the FP64 difference (double layer and combined field):
+    // for close points on a smooth surface it is far smaller than |source - target| and loses
+    // its leading digits if formed from the rounded difference
     const R d2 = x0 * x0 + x1 * x1 + x2 * x2;
     const R inv4pi = R(INV_4PI_D);
     if constexpr (KIND == KH || KIND == KM || (KIND == KG))
@@ -646,11 +648,10 @@ __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const 
             return {0, 0};
         R d, invd;
         dist_inv(d2, d, invd);
-        const R xn = x0 * R(nrm[0]) + x1 * R(nrm[1]) + x2 * R(nrm[2]);
         Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
         Cx<R> mk = {R(-kp.k_re) - invd, R(-kp.k_im)};
         Cx<R> v = cmul(e, mk);
-        return cmul(w, cscale(v, -inv4pi * (invd * invd) * xn));
+        return cmul(w, cscale(v, -inv4pi * (invd * invd) * R(xn)));
     }
     else // KC: combined_field_helmholtz_ifgf.hpp:43-68
     {
@@ -658,7 +659,7 @@ __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, const 
         if (!(d2 < R(1e-14)))
             dist_inv(d2, d, invd);
         const R dd = d2 * invd;
-        const R nxy = -(R(nrm[0]) * x0 + R(nrm[1]) * x1 + R(nrm[2]) * x2);
+        const R nxy = R(-xn);
         const R kap = R(kp.kappa);
         R s, c;
         sincos_r<R>(kap * dd, &s, &c);

@@ -124,9 +124,11 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
             __syncthreads();
             for (int k = grp; k < n; k += ngrp)
             {
-                const R x0 = R(sp[3 * k] - y0), x1 = R(sp[3 * k + 1] - y1),
-                        x2 = R(sp[3 * k + 2] - y2);
-                acc = cadd(acc, p2p<KIND, R>(kb, x0, x1, x2, NRM ? sn + 3 * k : nullptr, sw[k]));
+                const double d0 = sp[3 * k] - y0, d1 = sp[3 * k + 1] - y1, d2 = sp[3 * k + 2] - y2;
+                double xn = 0;
+                if constexpr (NRM)
+                    xn = d0 * sn[3 * k] + d1 * sn[3 * k + 1] + d2 * sn[3 * k + 2];
+                acc = cadd(acc, p2p<KIND, R>(kb, R(d0), R(d1), R(d2), xn, sw[k]));
             }
         }
     }
