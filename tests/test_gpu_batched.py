@@ -165,6 +165,26 @@ def test_merged_geometries_match_separate_operators(gpu, prec):
     assert rel(us[2][idx], d) < 2e-3
 
 
+def test_merged_tolerance_order_class(gpu):
+    """Two geometries (one the other scaled by 1.5, kappa by 1/1.5) whose tolerance estimates
+    give the orders (6,8,7)/(3,5,4), merged into one operator: the LDS-staged upward pass reads
+    each part's lists of child cones (mixed form), each part against its own operator."""
+    x = sphere(12000, 100 + 12000 % 97)
+    geos = [(x, 16.0), (x * 1.5, 16.0 / 1.5)]
+    ops, ws = [], []
+    for i, (p, k) in enumerate(geos):
+        op = ifgf_amd.HelmholtzIfgfOperator(k, 64, 6, 1, 1e-5)
+        op.init(p.T, p.T)
+        o = op.plan_orders()
+        assert o["high"].tolist() == [6, 8, 7] and o["low"].tolist() == [3, 5, 4], o
+        ops.append(op)
+        ws.append(cdens(len(p), 130 + i))
+    m = ifgf_amd.merge(ops)
+    us = m.mult(ws)
+    for op, w, u in zip(ops, ws, us):
+        assert rel(u, op.mult(w)) < TOL_SAME["mixed"]
+
+
 def test_configuration3_sixteen_spheres_one_launch_per_level(gpu):
     """16 spheres, N log-uniform in [4096, 262144], kappa uniform in [8, 64], merged into one
     operator (mixed form): every part against its own operator; timings of one merged mult
