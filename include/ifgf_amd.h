@@ -134,7 +134,7 @@ int ifgf_last_timings(const ifgf_operator* op, double* ms6);
 /* --- launch shapes (no counterpart in the reference: its TBB loops have no tile shapes) ---
  * shape: IFGF_LAUNCH_SHAPE_FIELDS ints
  *         [0] XCD run log2 (work items dealt to the 8 XCDs in runs of 2^v; 0 = plain order)
- *         [1] transform cones per workgroup (0 = default; orders 6 and 8 only)
+ *         [1] transform cones per workgroup (0 = default; the compiled order classes only)
  *         [2] upward parent cones per workgroup (2, 4, 8, 16; default 16)
  *         [3] batched columns per pass (0 = by nrhs, 8, 32)
  *         [4] mixed far field: waves per workgroup sharing the staged angle tables (1, 2, 4, 8)
