@@ -53,6 +53,15 @@ def direct_sum_torch(x, w, idx, kappa, dev):
 KERNEL_OF_PHASE = {"near": ("k_near",), "leaf": ("k_leaf_box", "k_leaf"),
                    "upward": ("k_up_lds", "k_up_cell"), "transform": ("k_transform_t",),
                    "far": ("k_far",)}
+# what bounds each phase's kernel below the FP32 vector roofline, from the rocprofv3 SQ
+# instruction and HBM counters of the 1M-point workloads (profiles/)
+LIMITER = {
+    "near": "VALU (issue rate near the practical ceiling)",
+    "leaf": "VALU (issue rate near the practical ceiling)",
+    "upward": "LDS reads of the staged child coefficients (one block per evaluation), then VALU",
+    "transform": "LDS traffic between the axis stages and the coefficient writes to HBM",
+    "far": "VALU in the FP64 cone location, and vector-memory latency of the coefficient gathers",
+}
 # rocprofv3 counter summaries of the 1M-point workloads, by tolerance
 COUNTERS = {
     1e-5: os.path.join(HERE, "profiles",
@@ -98,26 +107,85 @@ def time_mults(op, w_dev, steps, warmup):
     return start.elapsed_time(end) / steps, u
 
 
-def cpu_baseline(x, w, kappa, order, leaf, tol, stride, nthreads):
-    """FP64 CPU restatement of the reference (the oracle) on every stride-th level-2 subtree."""
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+# the CPU share of one GPU on the benchmark hosts: the baseline uses at most this many threads
+CPU_SHARE = 16
+
+
+def cpu_baseline(x, w, kappa, order, leaf, tol, stride, validate_n=0):
+    """The FP64 CPU restatement of the reference (the oracle, std::thread over the host's
+    cores) on every stride-th subtree of the first level with more than 32 boxes, scaled by
+    the point fraction; built -O3 for x86-64-v3 with -ffast-math (as the reference builds
+    itself) and without (IEEE).  validate_n > 0: also time full mults at validate_n points
+    (kappa scaled by sqrt(validate_n / N), the same points per wavelength) against the
+    extrapolated sample, to check the scaling."""
     from oracle import pyoracle as po
-    orc = po.Oracle(po.HELMHOLTZ, kappa, leaf, order, 1, tol)
-    t0 = time.time()
-    orc.init(x, x, nthreads=nthreads)
-    t_init = time.time() - t0
-    t0 = time.time()
-    frac = orc.mult_sample(w, stride, nthreads=nthreads)
-    t = time.time() - t0
-    full = t / max(frac, 1e-12)
-    return {
+    model, ncpu, avail = host_cpu()
+    nthreads = max(1, min(CPU_SHARE, avail))
+    res = {}
+    for variant in ("_fast", "_o3"):
+        orc = po.Oracle(po.HELMHOLTZ, kappa, leaf, order, 1, tol, variant=variant)
+        t0 = time.time()
+        orc.init(x, x, nthreads=nthreads)
+        t_init = time.time() - t0
+        orc.mult_sample(w, stride * 4, nthreads=nthreads)  # warm-up (page faults, caches)
+        t0 = time.time()
+        frac = orc.mult_sample(w, stride, nthreads=nthreads)
+        t = time.time() - t0
+        res[variant] = (t / max(frac, 1e-12), frac, t, t_init)
+        del orc
+    full, frac, t, t_init = res["_fast"]
+    out = {
         "value": 1.0 / full,
         "unit": "matvecs/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": (f"recursive phase of every {stride}th subtree at the first level with >32 "
-                   f"boxes ({frac * 100:.1f}% of the source points) timed {t:.1f}s and scaled "
-                   f"by the point fraction; plan build {t_init:.1f}s untimed"),
+        "sample": (f"FP64 oracle -O3 -march=x86-64-v3 -ffast-math, {nthreads} threads: the "
+                   f"recursive phase of every {stride}th subtree at the first level with >32 "
+                   f"boxes ({frac * 100:.1f}% of the source points) took {t:.1f}s, scaled by "
+                   f"the point fraction; plan build {t_init:.1f}s untimed"),
+        "ieee_o3_value": 1.0 / res["_o3"][0],
+        "cpu_model": model,
+        "nproc": ncpu,
+        "cpus_available": avail,
     }
+    if validate_n > 0:
+        n = int(validate_n)
+        kv = kappa * np.sqrt(n / len(x))
+        xv = sphere(n, 1099)
+        wv = np.random.default_rng(2099).standard_normal(n) + \
+            1j * np.random.default_rng(2199).standard_normal(n)
+        orc = po.Oracle(po.HELMHOLTZ, kv, leaf, order, 1, tol, variant="_fast")
+        orc.init(xv, xv, nthreads=nthreads)
+        orc.mult(wv, nthreads=nthreads)
+        ts = []
+        for _ in range(3):
+            t0 = time.time()
+            orc.mult(wv, nthreads=nthreads)
+            ts.append(time.time() - t0)
+        t0 = time.time()
+        fv = orc.mult_sample(wv, stride, nthreads=nthreads)
+        tsv = time.time() - t0
+        out["validation"] = {"n_points": n, "kappa": kv, "full_mult_s_median": float(np.median(ts)),
+                             "extrapolated_s": tsv / max(fv, 1e-12), "sample_fraction": fv}
+    return out
 
 
 def main():
@@ -137,6 +205,8 @@ def main():
     ap.add_argument("--samples", type=int, default=1000)
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-validate", type=int, default=0, metavar="N",
+                    help="also time full CPU mults at N points against the extrapolated sample")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no tuning after init)")
@@ -280,7 +350,8 @@ def main():
     if kc:
         traffic = (kc["fetch_bytes_per_mult"] + kc["write_bytes_per_mult"]) / kc["launches_per_mult"]
     roofline = {
-        "bound": "mfma",
+        "bound": "valu",
+        "limiter": LIMITER[dom],
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
@@ -288,9 +359,9 @@ def main():
         "traffic": traffic,
         "kernel": dom,
         "launches_per_mult": kc["launches_per_mult"] if kc else None,
-        "note": "compute-bound on the vector ALUs; peak = MI355X FP32 157.3 TFLOP/s (vector with "
-                "packed FP32, and matrix); the kernels are built without packed FP32 "
-                "instructions, so their vector ceiling is half of it: algorithmic flops of the "
+        "note": "no kernel issues MFMA; the roofline is the FP32 vector rate, peak = MI355X FP32 "
+                "157.3 TFLOP/s (with packed FP32; the kernels are built without packed FP32 "
+                "instructions, so their vector ceiling is 78.6): algorithmic flops of the "
                 "phase per mult / its device time "
                 "(= per launch / mean launch duration); traffic = HBM bytes per launch from "
                 "rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a plain copy (" +
@@ -302,9 +373,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        nthreads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(x, w, args.kappa, args.order, args.leaf, args.tol, args.cpu_stride,
-                           nthreads)
+                           args.cpu_validate)
 
     line = {
         "metric": "IFGF matvecs/sec at N=1M, kappa=100, rel.err<=1e-4 vs FP64 direct; "

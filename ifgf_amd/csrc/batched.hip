@@ -102,7 +102,7 @@ __global__ __launch_bounds__(64) void k_near_b(const WorkItem* __restrict__ item
     __shared__ double sp[3 * 64];
     __shared__ double sn[NRM ? 3 * 64 : 1];
     __shared__ Cx<R> sw[64 * RB];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(128) void k_leaf_b(const int32_t* __restrict__ work
     __shared__ R sx[CH], sy[CH], sz[CH];
     __shared__ R nx[CH], ny[CH], nz[CH];
     __shared__ Cx<R> sw[CH * RB];
-    const int64_t wi = xcd_block();
+    const int64_t wi = xcd_block(kp.xlg);
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_up_cell_b(const WorkItem* __restrict__ 
 {
     __shared__ int64_t toff[K][8];
     __shared__ int32_t gs[K];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
                                                     const UpEnt<R>* __restrict__ ents,
                                                     const int64_t* __restrict__ part_ent,
                                                     const Cx<R>* __restrict__ F,
-                                                    Cx<R>* __restrict__ V)
+                                                    Cx<R>* __restrict__ V, int xlg)
 {
     constexpr int N = O0 * O1 * O2;
     constexpr int TQ = 256 / RB;             // node lanes
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_up_stage_b(const WorkItem* __restrict__
     __shared__ int32_t s_qh[NH];             // its hash position
     __shared__ R s_qx[3][NH];
     __shared__ Cx<R> s_qtf[NH];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_up_b(const int32_t* __restrict__ work, 
                                               const Cx<R>* __restrict__ F,
                                               Cx<R>* __restrict__ V, KParams kp)
 {
-    const int64_t wi = xcd_block();
+    const int64_t wi = xcd_block(kp.xlg);
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(256) void k_transform_b(int64_t c0, int64_t c1,
                                                      const ConeDev* __restrict__ cones,
                                                      const Cx<R>* __restrict__ V,
                                                      Cx<R>* __restrict__ Fout,
-                                                     const R* __restrict__ W)
+                                                     const R* __restrict__ W, int xlg)
 {
     static_assert(RB % NI == 0, "instances of a workgroup share one cone");
     constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_transform_b(int64_t c0, int64_t c1,
     __shared__ unsigned char subl[32];
     __shared__ int nsub;
     const int tid = threadIdx.x;
-    const int64_t inst0 = xcd_block() * NI; // first (cone, column) instance
+    const int64_t inst0 = xcd_block(xlg) * NI; // first (cone, column) instance
     const int64_t cone = c0 + inst0 / RB;
     const int r0 = int(inst0 % RB);
     if (cone >= c1)
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(256) void k_far_b(const WorkItem* __restrict__ item
     __shared__ Cx<R> s_cf[4][64];
     constexpr int TL = 256 / RB;               // target lanes of the evaluation phase
     constexpr int TPT = (64 + TL - 1) / TL;    // targets per thread
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -841,7 +841,8 @@ void launch_transform_b(DevicePlan& D, hipStream_t st, int64_t c0, int64_t c1, c
     const R* W = static_cast<const R*>(D.wmat);
     const int64_t blocks = (c1 - c0) * RB / NI;
     k_transform_b<R, H0, H1, H2, L0, L1, L2, 3, 3, 3, NI, RB>
-        <<<dim3(unsigned(blocks)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W);
+        <<<dim3(unsigned(blocks)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W,
+                                                       D.shape.xcd_log2);
 }
 
 template <int KIND, typename R, int RB, int O0, int O1, int O2>
@@ -898,7 +899,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
 #define IFGF_UP_STAGE_B(K)                                                                      \
     k_up_stage_b<R, O0, O1, O2, 108, K, RB><<<dim3(unsigned(i1 - i0)), dim3(256), 0, st>>>(    \
         D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[l], D.cones, D.child, D.geo, D.ctab, \
-        static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V)
+        static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp.xlg)
             switch (D.shape.up_k)
             {
             case 2: // items of at most 2 cones run as items of 4
@@ -983,8 +984,6 @@ void run_order(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t 
         run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io);
 }
 } // namespace
-
-void set_xcd_log2_batched(int lg) { dev::set_xcd_log2(lg); }
 
 bool batched_supported(const DevicePlan& D)
 {

@@ -88,6 +88,8 @@ extern "C"
             return fail(op, IFGF_EINVAL, "setDx applies to the gradient kernel, dx in -1..2");
         op->K.dx = dx;
         op->plan.K.dx = dx;
+        if (op->dev.dx != dx)
+            device_drop_graph(op->dev); // the captured kernels carry the old direction
         op->dev.dx = dx;
         return IFGF_OK;
     }

@@ -44,6 +44,7 @@ struct KParams
     int lo0, lo1, lo2;
     int f0, f1, f2;
     int n_hi, n_lo;
+    int xlg; // XCD run length log2 of the launch (xcd_block)
 };
 
 constexpr double PI_D = 3.14159265358979323846;
@@ -740,33 +741,20 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
 // groups splitting the interaction list; reduction over groups by xor shuffles.
 // Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8) and every XCD has its
 // own L2.  Neighbouring work items read much of the same data (the same source boxes, the same
-// children, the same cells), so each XCD takes runs of 2^c_xcd_log2 consecutive items, the
-// runs dealt out round-robin so that costs drifting along a work list stay balanced: workgroup
-// b handles item xcd_block(), a bijection of [0, gridDim.x) for every run length (the tail
-// beyond the last whole round of runs keeps its order; run length 1 is the plain order).  The
-// run length is a launch shape the autotuner picks: each translation unit holds its own copy,
-// set from the host by set_xcd_log2 before a mult.
-static __constant__ int c_xcd_log2 = 4;
-__device__ __forceinline__ int64_t xcd_block()
+// children, the same cells), so each XCD takes runs of 2^lg consecutive items, the runs dealt
+// out round-robin so that costs drifting along a work list stay balanced: workgroup b handles
+// item xcd_block(lg), a bijection of [0, gridDim.x) for every run length (the tail beyond the
+// last whole round of runs keeps its order; run length 1 is the plain order).  The run length
+// is a launch shape the autotuner picks per operator; it travels with each launch (KParams::xlg
+// or a kernel argument), so launches of operators with different shapes never share state.
+__device__ __forceinline__ int64_t xcd_block(int lg)
 {
     const int64_t n = gridDim.x, b = blockIdx.x;
-    const int lg = c_xcd_log2;
     const int64_t full = (n >> (3 + lg)) << (3 + lg);
     if (b >= full)
         return b;
     const int64_t x = b & 7, i = b >> 3;
     return ((((i >> lg) << 3) + x) << lg) + (i & ((int64_t(1) << lg) - 1));
-}
-
-// host side of one translation unit's run length (the constant is written only on a change)
-static int h_xcd_log2 = 4;
-static inline void set_xcd_log2(int lg)
-{
-    if (lg == h_xcd_log2)
-        return;
-    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_xcd_log2), &lg, sizeof(int), 0,
-                             hipMemcpyHostToDevice));
-    h_xcd_log2 = lg;
 }
 
 // The near field and the far groups of a mult on the plan's side stream (host side).  The
@@ -898,6 +886,7 @@ inline KParams kparams(const DevicePlan& D)
     k.f2 = D.f[2];
     k.n_hi = D.n_hi;
     k.n_lo = D.n_lo;
+    k.xlg = D.shape.xcd_log2;
     return k;
 }
 

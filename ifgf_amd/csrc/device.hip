@@ -85,7 +85,7 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
     __shared__ double sp[3 * 64];
     __shared__ double sn[NRM ? 3 * 64 : 1];
     __shared__ Cx<R> sw[64];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ i
     __shared__ R sx[CH], sy[CH], sz[CH];
     __shared__ R nx[CH], ny[CH], nz[CH];
     __shared__ Cx<R> sw[CH];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it]; // box, its first cone, its number of cones
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(128) void k_up(const int32_t* __restrict__ work, in
                                             const Cx<R>* __restrict__ F, Cx<R>* __restrict__ V,
                                             KParams kp)
 {
-    const int64_t wi = xcd_block();
+    const int64_t wi = xcd_block(kp.xlg);
     if (wi >= nwork)
         return;
     const int32_t g = work[wi];
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
 {
     __shared__ int64_t toff[K][8]; // table offset of child c of cone k; -1: no child
     __shared__ int32_t gs[K];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k
     // whole item up front so that each staging step waits for one load only
     __shared__ int32_t sslot[K][8][DMAX];
     __shared__ int32_t dcnt[8];
-    const int64_t it = xcd_block();
+    const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
                                                      const ConeDev* __restrict__ cones,
                                                      const Cx<R>* __restrict__ V,
                                                      Cx<R>* __restrict__ Fout,
-                                                     const R* __restrict__ W)
+                                                     const R* __restrict__ W, int xlg)
 {
     constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
     constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2, NSUB = F0 * F1 * F2;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     __shared__ uint32_t cmask[CPB];
     __shared__ int cbase[CPB];
     const int tid = threadIdx.x;
-    const int64_t first = c0 + xcd_block() * CPB;
+    const int64_t first = c0 + xcd_block(xlg) * CPB;
     const int ncone = int(min<int64_t>(CPB, c1 - first));
     for (int e = tid; e < ncone * NH; e += 256)
     {
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
     Cx<R>* Y = X + int64_t(cpb) * xsz;
     for (int i = threadIdx.x; i < nW; i += blockDim.x)
         Ws[i] = W[i];
-    const int64_t first = c0 + xcd_block() * cpb;
+    const int64_t first = c0 + xcd_block(kp.xlg) * cpb;
     const int ncone = int(min<int64_t>(cpb, c1 - first));
     const int nh = h0 * h1 * h2;
     for (int e = threadIdx.x; e < ncone * nh; e += blockDim.x)
@@ -731,7 +731,7 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
     // workgroup: the tables, not the registers, would otherwise bound the waves per CU)
     const int lane = WPB == 1 ? int(threadIdx.x) : int(threadIdx.x & 63);
     const int64_t it =
-        WPB == 1 ? xcd_block() : xcd_block() * WPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        WPB == 1 ? xcd_block(kp.xlg) : xcd_block(kp.xlg) * WPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (WPB == 1 && it >= nitems)
         return;
     if constexpr (FAST)
@@ -846,12 +846,6 @@ static std::vector<WorkItem> make_items(const std::vector<int32_t>& boxes, const
 }
 
 
-struct Timing
-{
-    float ms[6] = {0, 0, 0, 0, 0, 0};
-};
-Timing g_last;
-bool g_force_timing = false; // phase timing on (the autotuner reads single phases)
 
 
 // up_level >= 0: the upward pass into parent level up_level; otherwise the far field of
@@ -975,7 +969,8 @@ void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t 
     const int cls = order_class(D);
 #define IFGF_TRANSFORM(H0, H1, H2, L0, L1, L2, CPB)                                            \
     k_transform_t<R, H0, H1, H2, L0, L1, L2, 3, 3, 3, CPB>                                      \
-        <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W)
+        <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W, \
+                                                                          kp.xlg)
     const int cpb_sel =
         D.shape.cpb > 0 ? D.shape.cpb : transform_cpb_choices(cls, sizeof(R))[0];
     if (cls == 1)
@@ -1102,22 +1097,22 @@ struct PhaseTimer
         HIP_OK(hipEventRecord(e, st));
         segs.push_back({open_phase, {open, e}});
     }
-    void finish(Timing& out)
+    void finish(float* out)
     {
         if (!on || segs.empty())
             return;
         HIP_OK(hipEventSynchronize(segs.back().second.second));
-        for (float& m : out.ms)
-            m = 0;
+        for (int i = 0; i < 6; i++)
+            out[i] = 0;
         for (auto& sg : segs)
         {
             float x = 0;
             HIP_OK(hipEventElapsedTime(&x, sg.second.first, sg.second.second));
-            out.ms[sg.first] += x;
+            out[sg.first] += x;
         }
         float tot = 0;
         HIP_OK(hipEventElapsedTime(&tot, segs.front().second.first, segs.back().second.second));
-        out.ms[5] = tot;
+        out[5] = tot;
         for (auto& sg : segs)
         {
             (void)hipEventDestroy(sg.second.first);
@@ -1233,7 +1228,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
         res, D.tgt_perm, out_dev, D.ntgt, out_re, out_im);
     pt.end();
     HIP_OK(hipGetLastError());
-    pt.finish(g_last);
+    pt.finish(D.last_ms);
 }
 
 template <typename R>
@@ -1277,7 +1272,7 @@ void device_free(DevicePlan& D)
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
-                    D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out};
+                    D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out, D.h_in, D.h_out};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1768,9 +1763,7 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
     try
     {
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const bool timed = g_force_timing || std::getenv("IFGF_PHASE_TIMING") != nullptr;
-        dev::set_xcd_log2(D.shape.xcd_log2);
-        set_xcd_log2_batched(D.shape.xcd_log2);
+        const bool timed = D.force_timing || std::getenv("IFGF_PHASE_TIMING") != nullptr;
         if (D.shape.overlap && !D.side_stream)
         {
             // the side stream of the overlapped near and far fields and its events, created
@@ -1822,13 +1815,15 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 return "";
             }
         }
-        double* wbuf = nullptr;
-        double* obuf = nullptr;
-        if (!device_ptrs)
+        // host densities and results go through two device vectors kept with the plan
+        // (allocated on the first host-pointer mult, not per call)
+        if (!device_ptrs && !D.h_in)
         {
-            HIP_OK(hipMalloc(&wbuf, 16 * size_t(D.nsrc)));
-            HIP_OK(hipMalloc(&obuf, 16 * size_t(D.ntgt)));
+            HIP_OK(hipMalloc(&D.h_in, 16 * size_t(D.nsrc)));
+            HIP_OK(hipMalloc(&D.h_out, 16 * size_t(D.ntgt)));
         }
+        double* wbuf = D.h_in;
+        double* obuf = D.h_out;
         for (int64_t r = 0; r < nrhs; r++)
         {
             const double* wr = static_cast<const double*>(w) + 2 * r * D.nsrc;
@@ -1849,11 +1844,7 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 HIP_OK(hipMemcpyAsync(orr, obuf, 16 * size_t(D.ntgt), hipMemcpyDeviceToHost, st));
         }
         if (!device_ptrs)
-        {
             HIP_OK(hipStreamSynchronize(st));
-            HIP_OK(hipFree(wbuf));
-            HIP_OK(hipFree(obuf));
-        }
     }
     catch (const std::exception& e)
     {
@@ -1891,16 +1882,21 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
     D.shape = sh;
     D.up_items = D.up_items_k[kc];
     D.up_item_off = D.up_item_off_k[kc];
-    if (changed && D.graph_exec)
-    {
-        // the captured kernels have the old shapes: capture again on the next graph mult
-        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.graph_exec));
-        D.graph_exec = nullptr;
-        if (D.graph)
-            (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
-        D.graph = nullptr;
-    }
+    if (changed)
+        device_drop_graph(D); // the captured kernels have the old shapes
     return "";
+}
+
+void device_drop_graph(DevicePlan& D)
+{
+    // a captured graph holds the kernel arguments of its capture (launch shapes, the gradient
+    // direction): it is captured again on the next graph mult
+    if (D.graph_exec)
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.graph_exec));
+    D.graph_exec = nullptr;
+    if (D.graph)
+        (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
+    D.graph = nullptr;
 }
 
 std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best)
@@ -1941,17 +1937,17 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
             double t_best = 1e300;
             for (int r = -1; r < reps; r++)
             {
-                g_force_timing = phase >= 0;
+                D.force_timing = phase >= 0;
                 HIP_OK(hipEventRecord(e0, st));
                 e = device_mult(D, w, o, nrhs, true, st);
                 HIP_OK(hipEventRecord(e1, st));
                 HIP_OK(hipEventSynchronize(e1));
-                g_force_timing = false;
+                D.force_timing = false;
                 if (!e.empty())
                     throw std::runtime_error(e);
                 float ms = 0;
                 HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-                const double t = phase >= 0 ? double(g_last.ms[phase]) : double(ms);
+                const double t = phase >= 0 ? double(D.last_ms[phase]) : double(ms);
                 if (r >= 0)
                     t_best = std::min(t_best, t);
             }
@@ -1994,7 +1990,7 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
     catch (const std::exception& ex)
     {
         err = std::string("autotune: ") + ex.what();
-        g_force_timing = false;
+        D.force_timing = false;
     }
     if (e0)
         (void)hipEventDestroy(e0);
@@ -2009,10 +2005,10 @@ std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& 
     return err;
 }
 
-void device_last_timings(const DevicePlan&, double* out6)
+void device_last_timings(const DevicePlan& D, double* out6)
 {
     for (int i = 0; i < 6; i++)
-        out6[i] = g_last.ms[i];
+        out6[i] = D.last_ms[i];
 }
 
 } // namespace ifgf

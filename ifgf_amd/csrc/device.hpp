@@ -172,6 +172,13 @@ struct DevicePlan
     std::vector<void*> ev_level; // per source level: its coefficients are written
 
     LaunchShape shape;
+    // per-phase device times of this operator's last timed mult (ms: near, leaf, upward,
+    // transform, far, total); force_timing: time the phases of the next mults (autotuner)
+    float last_ms[6] = {0, 0, 0, 0, 0, 0};
+    bool force_timing = false;
+
+    // staging of host densities and results of single-column mults (first use)
+    double *h_in = nullptr, *h_out = nullptr;
 
     int b_rb = 0;
     void *b_ws = nullptr, *b_res = nullptr, *b_V = nullptr, *b_F = nullptr;
@@ -199,13 +206,14 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                         void* stream, int io = 0, bool graph = false);
 // Apply a launch shape (validated; an unknown choice is an error).  Drops a captured graph.
 std::string device_set_shape(DevicePlan& D, const LaunchShape& s);
+// Drop the captured graph of the single-column mult (after a change of its kernel arguments).
+void device_drop_graph(DevicePlan& D);
 // Time the candidate shapes on this operator with synthetic densities (nrhs columns, each
 // candidate `reps` times, the fastest kept), one choice after another: the XCD run length on
 // the whole mult, then the transform's cones per workgroup, the upward cones per item and the
 // far field's waves per workgroup on their phases, then (nrhs >= 2) the batched columns per
 // pass.  Leaves the best shape applied.
 std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best);
-void set_xcd_log2_batched(int lg);
 // per-phase timings of the last mult (ms): near, leaf, upward, transform, far, total
 void device_last_timings(const DevicePlan& D, double* out6);
 bool device_available();

@@ -45,7 +45,9 @@ extern "C" {
 /* ifgf_mult flags */
 #define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */
 #define IFGF_MULT_GRAPH 2      /* one column: replay the mult's kernels as a captured HIP graph
-                                  (captured on the first such call; reused while the operator lives) */
+                                  (captured on the first such call and reused; captured again
+                                  after ifgf_set_launch_shape / ifgf_autotune change a shape or
+                                  ifgf_set_dx changes the direction) */
 
 typedef struct ifgf_operator ifgf_operator;
 
@@ -127,8 +129,9 @@ int ifgf_plan_cones(const ifgf_operator* op, int64_t* nc, int64_t* nf, int64_t* 
  * [7] n_hi  [8] n_lo  [9] transform flops per coarse cone */
 int ifgf_plan_work(const ifgf_operator* op, double* out10);
 
-/* per-phase device times of the last mult in ms (near, leaf, upward, transform, far, total);
- * filled only when the environment variable IFGF_PHASE_TIMING is set */
+/* per-phase device times of this operator's last mult in ms (near, leaf, upward, transform,
+ * far, total); filled only when the environment variable IFGF_PHASE_TIMING is set (adjoint
+ * products of other targets run on the transposed operator and do not update them) */
 int ifgf_last_timings(const ifgf_operator* op, double* ms6);
 
 /* --- launch shapes (no counterpart in the reference: its TBB loops have no tile shapes) ---

@@ -956,9 +956,13 @@ struct Operator
             base_order[d] = best[d];
     }
 
-    void estimate_on_box(int level, int id, int out[3]) // ifgfoperator.hpp:143-244
+    // one sampling step of estimateRefinementOnBox (ifgfoperator.hpp:181-217): the factored
+    // kernel of the box's sources with unit weights at the Chebyshev nodes of `order` in
+    // element 0 of the box's coarse grid, and its Chebyshev coefficients.  ypts: the nodes in
+    // Cartesian coordinates; info: H, centre (3), smin, the grid's elements (3)
+    void estimate_sample(int level, int id, const int order[3], std::vector<double>& ypts,
+                         std::vector<cplx>& data, std::vector<cplx>& trafo, double info[8])
     {
-        const int maxR = 10;
         const Node& nd = S.lv[level][id];
         const double H = nd.bb.side;
         const double* xc = nd.bb.c;
@@ -966,11 +970,55 @@ struct Operator
         double smax = std::sqrt(3.0) / 3;
         double smin = K.cutoff_limit(H);
         double blo[3] = {smin, 0, -PI}, bhi[3] = {smax, PI, PI};
+        size_t nels[3];
+        K.elements_for_box(H, base_els, 1, nels);
+        ConeGrid g;
+        g.setup(nels, blo, bhi);
+        auto nodes = chebnodes3d(order);
+        const size_t nn = nodes.size() / 3;
+        ypts.assign(3 * nn, 0.0);
+        data.assign(nn, cplx(0, 0));
+        trafo.assign(nn, cplx(0, 0));
+        // the sampled element, (n_elements * Random()) / SIZE_MAX in the reference (:218),
+        // evaluates to 0
+        const size_t el = 0;
+        for (size_t q = 0; q < nn; q++)
+        {
+            double sp[3], y[3];
+            g.transform(el, &nodes[3 * q], sp);
+            interp_to_cart(sp, xc, H, y);
+            double dv[3] = {y[0] - xc[0], y[1] - xc[1], y[2] - xc[2]};
+            double dc = norm3(dv);
+            cplx acc = 0;
+            for (int64_t i = 0; i < nS; i++)
+                acc += K.factored(&S.pts[3 * (nd.begin + i)], y, cplx(1.0, 0.0), dc,
+                                  nd.begin + i);
+            data[q] = acc;
+            for (int d = 0; d < 3; d++)
+                ypts[3 * q + d] = y[d];
+        }
+        chebtransform3(data.data(), trafo.data(), order);
+        info[0] = H;
+        for (int d = 0; d < 3; d++)
+        {
+            info[1 + d] = xc[d];
+            info[5 + d] = double(nels[d]);
+        }
+        info[4] = smin;
+    }
+
+    void estimate_on_box(int level, int id, int out[3]) // ifgfoperator.hpp:143-244
+    {
+        const int maxR = 10;
+        const double H = S.lv[level][id].bb.side;
         int base[3] = {0, 0, 0};
         double err[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
                          std::numeric_limits<double>::max()};
         int new_p[3] = {base_order[0], base_order[1], base_order[2]};
         auto maxc = [](const double* e) { return std::max(e[0], std::max(e[1], e[2])); };
+        std::vector<double> ypts;
+        std::vector<cplx> data, trafo;
+        double info[8];
         while (maxc(err) > K.tol && std::max(base[0], std::max(base[1], base[2])) < maxR)
         {
             for (int d = 0; d < 3; d++)
@@ -979,30 +1027,9 @@ struct Operator
             int plus[3] = {new_p[0] + layers + 1, new_p[1] + layers + 1, new_p[2] + layers + 1};
             int order[3];
             K.order_for_box(H, plus, 1, order);
-            size_t nels[3];
-            K.elements_for_box(H, base_els, 1, nels);
-            ConeGrid g;
-            g.setup(nels, blo, bhi);
-            auto nodes = chebnodes3d(order);
-            const size_t nn = nodes.size() / 3;
-            std::vector<cplx> data(nn), trafo(nn);
-            const size_t el = 0;
-            for (size_t q = 0; q < nn; q++)
-            {
-                double sp[3], y[3];
-                g.transform(el, &nodes[3 * q], sp);
-                interp_to_cart(sp, xc, H, y);
-                double dv[3] = {y[0] - xc[0], y[1] - xc[1], y[2] - xc[2]};
-                double dc = norm3(dv);
-                cplx acc = 0;
-                for (int64_t i = 0; i < nS; i++)
-                    acc += K.factored(&S.pts[3 * (nd.begin + i)], y, cplx(1.0, 0.0), dc,
-                                      nd.begin + i);
-                data[q] = acc;
-            }
-            chebtransform3(data.data(), trafo.data(), order);
+            estimate_sample(level, id, order, ypts, data, trafo, info);
             for (int d = 0; d < 3; d++)
-                err[d] = compute_slice_norm(trafo.data(), nn, order, d, layers);
+                err[d] = compute_slice_norm(trafo.data(), trafo.size(), order, d, layers);
             for (int d = 0; d < 3; d++)
                 if (err[d] > K.tol)
                     base[d] += 1;
@@ -1639,6 +1666,29 @@ extern "C"
         auto* h = static_cast<OrcHandle*>(hv);
         h->op->nthreads = nthreads;
         return h->op->mult_sample(reinterpret_cast<const cplx*>(w), stride);
+    }
+
+    // util.hpp:214-234 on an interleaved complex array of rows = n0 n1 n2 values
+    double orc_slice_norm(const double* data, int64_t rows, const int* ns, int axis, int layers)
+    {
+        return compute_slice_norm(reinterpret_cast<const cplx*>(data), size_t(rows), ns, axis,
+                                  layers);
+    }
+
+    // one sampling step of the order estimate on source box `id` of `level` at `order`:
+    // nodes (n x 3 Cartesian), samples and coefficients (n complex), info (8, as
+    // Operator::estimate_sample); returns n = order[0] order[1] order[2]
+    int64_t orc_estimate_sample(void* hv, int level, int64_t id, const int* order, double* pts,
+                                double* data, double* trafo, double* info)
+    {
+        auto* h = static_cast<OrcHandle*>(hv);
+        std::vector<double> y;
+        std::vector<cplx> d, t;
+        h->op->estimate_sample(level, int(id), order, y, d, t, info);
+        std::memcpy(pts, y.data(), y.size() * sizeof(double));
+        std::memcpy(data, d.data(), d.size() * sizeof(cplx));
+        std::memcpy(trafo, t.data(), t.size() * sizeof(cplx));
+        return int64_t(d.size());
     }
 
     void orc_base_order(void* hv, int* out)

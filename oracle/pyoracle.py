@@ -15,19 +15,25 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 
 HELMHOLTZ, MODHELMHOLTZ, LAPLACE, DOUBLELAYER, COMBINED, GRAD = range(6)
 
-_lib = None
+_libs = {}
+_lib = None  # the IEEE checker, once loaded
+
+# builds of the same source (oracle/Makefile): "" the IEEE checker of the tests, "_o3" and
+# "_fast" (-O3 for x86-64-v3, IEEE / -ffast-math) for bench.py's CPU baseline
+VARIANTS = ("", "_o3", "_fast")
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def lib():
+def lib(variant=""):
     global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+    if variant not in _libs:
+        path = os.path.join(_HERE, f"liboracle{variant}.so")
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(_LIB)
+        L = ctypes.CDLL(path)
         P, D, I, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int64
         L.orc_create.restype = P
         L.orc_create.argtypes = [I, D, D, I64, I, I, D, I]
@@ -61,8 +67,16 @@ def lib():
         L.orc_interp_to_cart.argtypes = [P, P, D, P]
         L.orc_kernel_eval.argtypes = [P, P, P, I64, P]
         L.orc_direct.argtypes = [P, P, I64, P, P, P, P, I64, P, I]
-        _lib = L
-    return _lib
+        L.orc_slice_norm.restype = D
+        L.orc_slice_norm.argtypes = [P, I64, P, I, I]
+        L.orc_estimate_sample.restype = I64
+        L.orc_estimate_sample.argtypes = [P, I, I64, P, P, P, P, P]
+        _libs[variant] = L
+        if variant == "":
+            _lib = L
+    return _libs[variant]
+
+
 
 
 def _p(a):
@@ -84,23 +98,24 @@ def default_threads():
 class Oracle:
     """CPU FP64 IFGF operator with the reference's construct/init/mult semantics."""
 
-    def __init__(self, kind, k, leaf, order, n_elem=1, tol=-1.0, dx=-1):
+    def __init__(self, kind, k, leaf, order, n_elem=1, tol=-1.0, dx=-1, variant=""):
         k = complex(k)
         self.kind = kind
-        self.h = lib().orc_create(kind, k.real, k.imag, int(leaf), int(order), int(n_elem),
+        self._L = lib(variant)
+        self.h = self._L.orc_create(kind, k.real, k.imag, int(leaf), int(order), int(n_elem),
                                   float(tol), int(dx))
 
     def __del__(self):
         try:
-            if getattr(self, "h", None) and _lib is not None:
-                _lib.orc_destroy(self.h)
+            if getattr(self, "h", None):
+                self._L.orc_destroy(self.h)
         except Exception:
             pass
         self.h = None
 
     def set_cutoff(self, dmax):
         """Drop admissible pairs with box distance > dmax at the next init (<= 0: keep all)."""
-        lib().orc_set_cutoff(self.h, float(dmax))
+        self._L.orc_set_cutoff(self.h, float(dmax))
         return self
 
     def init(self, srcs, tgts, normals=None, nthreads=None):
@@ -109,46 +124,46 @@ class Oracle:
         self._srcs, self._tgts = s, t
         n = None if normals is None else _pts(normals)
         self._normals = n
-        rc = lib().orc_init(self.h, _p(s), s.shape[0], _p(t), t.shape[0],
+        rc = self._L.orc_init(self.h, _p(s), s.shape[0], _p(t), t.shape[0],
                             None if n is None else _p(n), nthreads or default_threads())
         if rc != 0:
-            raise RuntimeError(lib().orc_last_error(self.h).decode())
+            raise RuntimeError(self._L.orc_last_error(self.h).decode())
         return self
 
     def mult(self, w, nthreads=None):
         w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
         out = np.zeros(self._tgts.shape[0], dtype=np.complex128)
-        rc = lib().orc_mult(self.h, _p(w), _p(out), nthreads or default_threads())
+        rc = self._L.orc_mult(self.h, _p(w), _p(out), nthreads or default_threads())
         if rc != 0:
-            raise RuntimeError(lib().orc_last_error(self.h).decode())
+            raise RuntimeError(self._L.orc_last_error(self.h).decode())
         return out
 
     def mult_sample(self, w, stride, nthreads=None):
         """Recursive phase on every stride-th first-recursive-level subtree; returns the
         fraction of source points covered (for a bounded CPU timing)."""
         w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
-        return lib().orc_mult_sample(self.h, _p(w), int(stride), nthreads or default_threads())
+        return self._L.orc_mult_sample(self.h, _p(w), int(stride), nthreads or default_threads())
 
     def base_order(self):
         o = np.zeros(3, dtype=np.int32)
-        lib().orc_base_order(self.h, _p(o))
+        self._L.orc_base_order(self.h, _p(o))
         return o
 
     def levels(self, tree=0):
-        return lib().orc_levels(self.h, tree)
+        return self._L.orc_levels(self.h, tree)
 
     def num_boxes(self, tree, level):
-        return lib().orc_num_boxes(self.h, tree, level)
+        return self._L.orc_num_boxes(self.h, tree, level)
 
     def perm(self, tree=0):
         n = (self._srcs if tree == 0 else self._tgts).shape[0]
         p = np.zeros(n, dtype=np.int64)
-        lib().orc_perm(self.h, tree, _p(p))
+        self._L.orc_perm(self.h, tree, _p(p))
         return p
 
     def root_box(self, tree=0):
         b = np.zeros(6)
-        lib().orc_root_box(self.h, tree, _p(b))
+        self._L.orc_root_box(self.h, tree, _p(b))
         return b
 
     def tree_level(self, tree, level):
@@ -158,33 +173,48 @@ class Oracle:
         parent = np.zeros(n, np.int32)
         leaf = np.zeros(n, np.int8)
         bbox = np.zeros((n, 6))
-        lib().orc_tree_level(self.h, tree, level, _p(begin), _p(end), _p(parent), _p(leaf),
+        self._L.orc_tree_level(self.h, tree, level, _p(begin), _p(end), _p(parent), _p(leaf),
                              _p(bbox))
         return dict(begin=begin, end=end, parent=parent, leaf=leaf.astype(bool), bbox=bbox)
 
     def interactions(self, level, far):
-        n = lib().orc_num_interactions(self.h, level, int(far))
+        n = self._L.orc_num_interactions(self.h, level, int(far))
         rows = np.zeros((n, 3), np.int64)
         if n:
-            lib().orc_interactions(self.h, level, int(far), _p(rows))
+            self._L.orc_interactions(self.h, level, int(far), _p(rows))
         return rows
 
     def cones(self, level, step):
         nb = self.num_boxes(0, level)
         nel = np.zeros((nb, 3), np.int64)
         nact = np.zeros(nb, np.int64)
-        lib().orc_cone_counts(self.h, level, step, _p(nel), _p(nact))
+        self._L.orc_cone_counts(self.h, level, step, _p(nel), _p(nact))
         ids = np.zeros(int(nact.sum()), np.int64)
         if ids.size:
-            lib().orc_cone_active(self.h, level, step, _p(ids))
+            self._L.orc_cone_active(self.h, level, step, _p(ids))
         return nel, nact, ids
 
+    def estimate_sample(self, level, box, order):
+        """One sampling step of the order estimate on source box `box` of `level` at the
+        Chebyshev orders `order`: (nodes (n, 3) Cartesian, samples (n,), coefficients (n,),
+        info dict with H, centre, smin and the grid's elements)."""
+        o = np.asarray(order, dtype=np.int32)
+        n = int(np.prod(o))
+        pts = np.zeros((n, 3))
+        data = np.zeros(n, np.complex128)
+        trafo = np.zeros(n, np.complex128)
+        info = np.zeros(8)
+        self._L.orc_estimate_sample(self.h, int(level), int(box), _p(o), _p(pts), _p(data),
+                                  _p(trafo), _p(info))
+        return pts, data, trafo, dict(H=info[0], centre=info[1:4].copy(), smin=info[4],
+                                      nels=info[5:8].astype(int))
+
     def min_rec(self):
-        return lib().orc_min_rec(self.h)
+        return self._L.orc_min_rec(self.h)
 
     def kernel(self, xs, y, src_idx=0):
         out = np.zeros(2)
-        lib().orc_kernel_eval(self.h, _p(np.ascontiguousarray(xs, np.float64)),
+        self._L.orc_kernel_eval(self.h, _p(np.ascontiguousarray(xs, np.float64)),
                               _p(np.ascontiguousarray(y, np.float64)), src_idx, _p(out))
         return complex(out[0], out[1])
 
@@ -196,7 +226,7 @@ class Oracle:
         w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
         idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
         out = np.zeros(idx.size, np.complex128)
-        lib().orc_direct(self.h, _p(s), s.shape[0], _p(w), None if n is None else _p(n), _p(t),
+        self._L.orc_direct(self.h, _p(s), s.shape[0], _p(w), None if n is None else _p(n), _p(t),
                          _p(idx), idx.size, _p(out), nthreads or default_threads())
         return out
 
@@ -206,10 +236,17 @@ class Oracle:
         idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
         out = np.zeros(idx.size, np.complex128)
         n = self._normals
-        lib().orc_direct(self.h, _p(self._srcs), self._srcs.shape[0], _p(w),
+        self._L.orc_direct(self.h, _p(self._srcs), self._srcs.shape[0], _p(w),
                          None if n is None else _p(n), _p(self._tgts), _p(idx), idx.size,
                          _p(out), nthreads or default_threads())
         return out
+
+
+def slice_norm(values, ns, axis, layers):
+    """util.hpp:214-234: relative norm of the coefficient slice index ns[axis] - layers."""
+    ns_a = np.asarray(ns, dtype=np.int32)
+    v = np.ascontiguousarray(values, dtype=np.complex128)
+    return lib().orc_slice_norm(_p(v), v.size, _p(ns_a), int(axis), int(layers))
 
 
 def chebnodes1d(n):

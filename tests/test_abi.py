@@ -59,3 +59,16 @@ def test_python_classes_mirror_reference_api():
     ifgf_amd.LaplaceIfgfOperator(64, 6, 1, -1.0)
     with pytest.raises(_lib.IfgfError):
         ifgf_amd.GradHelmholtzIfgfOperator(1 + 2j, 64, 6, 1, -1.0).setDx(3)
+
+
+def test_import_pyifgf_unchanged():
+    """`import pyifgf` (py_ifgf.cpp:25) works and gives the reference's classes, taking the
+    reference's positional arguments; a plan-only init and the plan's orders work on the CPU."""
+    import pyifgf
+    for name in ("MofifiedHelmholtzIfgfOperator", "GradHelmholtzIfgfOperator",
+                 "LaplaceIfgfOperator"):
+        assert getattr(pyifgf, name) is getattr(ifgf_amd, name)
+    op = pyifgf.MofifiedHelmholtzIfgfOperator(0.5 - 3.0j, 64, 8, 1, -1.0)
+    x = np.random.default_rng(1).standard_normal((3, 500))
+    op.init(x, x, plan_only=True)
+    assert op.plan_orders()["base"].tolist() == [5, 8, 8]

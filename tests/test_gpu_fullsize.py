@@ -45,6 +45,32 @@ def test_full_size_properties(gpu):
     assert err < 3e-3 and abs(err - err64) < 1e-4
 
 
+def test_headline_tolerance_path(gpu):
+    """The benchmarked workload: N = 1,048,576 on the unit sphere, kappa = 100, order 6,
+    leaf 64, tolerance 1e-5 (the order estimate raises the orders to (6,8,7), fine (3,5,4)),
+    mixed form: relative error <= 1e-4 against an FP64 direct sum on 1,000 sampled targets."""
+    import torch
+
+    from bench import direct_sum_torch
+
+    N = 1 << 20
+    x = sphere(N, 1002)
+    w = cdens(N, 2002)
+    dev = torch.device("cuda:0")
+    print("headline: N=1048576 kappa=100 order=6 tol=1e-5 mixed", flush=True)
+    op = ifgf_amd.HelmholtzIfgfOperator(100.0, 64, 6, 1, 1e-5, precision="mixed")
+    op.init(x.T, x.T)
+    o = op.plan_orders()
+    assert o["base"].tolist() == [6, 8, 7] and o["high"].tolist() == [6, 8, 7], o
+    assert o["low"].tolist() == [3, 5, 4], o
+    u = op.mult(torch.as_tensor(w, device=dev)).cpu().numpy()
+    idx = np.random.default_rng(3002).choice(N, 1000, replace=False)
+    ref = direct_sum_torch(x, w, idx, 100.0, dev)
+    err = np.linalg.norm(u[idx] - ref) / np.linalg.norm(ref)
+    print(f"headline: rel. error vs direct sum {err:.3e}", flush=True)
+    assert err <= 1e-4, err
+
+
 def test_combined_field_torus_mixed_vs_fp64(gpu):
     """Combined field on a torus (R=1, r=0.4, analytic normals), order 8, tolerance 1e-5 (the
     estimateRefinement path), N = 262,144, kappa = 64: mixed vs FP64 form and both vs an FP64
@@ -67,7 +93,7 @@ def test_combined_field_torus_mixed_vs_fp64(gpu):
     e_64 = np.linalg.norm(u["fp64"][idx] - ref) / np.linalg.norm(ref)
     d = np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"])
     print(f"torus: vs direct mixed {e_mix:.2e} fp64 {e_64:.2e}; mixed vs fp64 {d:.2e}", flush=True)
-    assert d < 1e-4 and e_64 < 1e-3 and e_mix < 1e-3
+    assert d < 1e-5 and e_64 < 1e-4 and e_mix < 1e-4
 
 
 @pytest.mark.timeout(1200)
@@ -100,4 +126,4 @@ def test_configuration5_torus_4M(gpu):
     ref = po.Oracle(po.COMBINED, kappa, 64, 8, 1, 1e-5).direct_sum(x, x, w, idx, normals=n)
     err = np.linalg.norm(u[idx] - ref) / np.linalg.norm(ref)
     print(f"configuration 5: rel. error vs direct sum {err:.2e}", flush=True)
-    assert err < 1e-3
+    assert err < 1e-4

@@ -222,3 +222,25 @@ def test_graph_replays_match_eager(gpu, prec):
             (w * 2).sum()
             torch.cuda.synchronize()
             assert torch.equal(u.cpu(), eager[i]), (rep, i)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_normals_kernels_match_direct_sum(gpu, prec):
+    """The HIP double-layer and combined-field mults at order 8 against FP64 direct sums from
+    the kernels' closed forms (tests/test_oracle_pins.py pins those sums by central
+    differences): independent of the oracle, so a sign or a dropped term in the factored
+    form, the transfer or the centre factor shared with the oracle fails here."""
+    from conftest import torus
+    from test_oracle_pins import direct_normals
+    x, n = torus(30000, 51)
+    w = cdens(30000, 52)
+    idx = np.arange(0, 30000, 97)
+    for kind, k in ((po.DOUBLELAYER, 0.5 + 20.0j), (po.COMBINED, 20.0)):
+        op = CLS[kind](k, 64, 8, 1, -1, precision=prec)
+        print(f"case normals vs direct sum kind={kind} k={k} prec={prec}", flush=True)
+        op.init(x.T, x.T, n.T)
+        u = op.mult(w)
+        d = direct_normals(kind, k, x, n, w, x, idx)
+        err = rel(u[idx], d)
+        print(f"normals kind={kind} {prec}: rel. error vs direct sum {err:.3e}", flush=True)
+        assert err < 3e-3, (kind, err)

@@ -123,3 +123,23 @@ def test_device_tree_full_size_sphere(gpu):
     cd, ch = dev.plan_counts(), host.plan_counts()
     for k in ("far_pairs", "near_pairs", "coarse_cones", "fine_cones"):
         assert cd[k] == ch[k], k
+
+
+def test_device_plan_matches_oracle_on_face_points(gpu):
+    """Points placed exactly on the child-box faces of the first three levels (the closed-box
+    scan, octree.hpp:1106-1199, and the Morton order decide every tie): the device-built plan
+    against the oracle directly (trees, interaction lists, cones), and its mult against the
+    oracle's."""
+    from oracle import pyoracle as po
+    from test_oracle_pins import faces_input
+    from test_plan import assert_same_plan
+    x, _ = faces_input()
+    print(f"device plan vs oracle: face points N={len(x)} leaf=16", flush=True)
+    op = ifgf_amd.HelmholtzIfgfOperator(4.0, 16, 6, 1, -1, precision="fp64")
+    op.init(x.T, x.T)
+    op._pts = (x, x)
+    orc = po.Oracle(po.HELMHOLTZ, 4.0, 16, 6).init(x, x)
+    assert_same_plan(op, orc)
+    w = cdens(len(x), 512)
+    u, ref = op.mult(w), orc.mult(w)
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 1e-11

@@ -224,3 +224,12 @@ def test_merged_plan_is_the_parts_side_by_side():
     other.init(geos[0][0].T, geos[0][0].T, plan_only=True)
     with pytest.raises(ifgf_amd.IfgfError):
         ifgf_amd.merge([ops[0], other], plan_only=True)
+
+
+def test_plan_matches_oracle_on_face_points():
+    """Points exactly on the child-box faces of the first three levels (the closed-box scan
+    and the Morton order decide every tie)."""
+    from test_oracle_pins import faces_input
+    x, _ = faces_input()
+    op, orc = make_pair("helmholtz", 4.0, 16, 6, 1, -1, x, x)
+    assert_same_plan(op, orc)
