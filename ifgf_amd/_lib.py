@@ -32,6 +32,7 @@ SIGNATURES = {
     "ifgf_set_dx": (_I, [_P, _I]),
     "ifgf_init": (_I, [_P, _P, _I64, _P, _I64, _P, CUTOFF_FN, _P, _I]),
     "ifgf_mult": (_I, [_P, _P, _I64, _P, _I, _P]),
+    "ifgf_mult_noalloc": (_I, [_P, _P, _P, _P]),
     "ifgf_mult_adjoint": (_I, [_P, _P, _I64, _P, _I, _P]),
     "ifgf_merge": (_I, [_P, _I, _I, ctypes.POINTER(_P)]),
     "ifgf_sizes": (_I, [_P, _P, _P]),

@@ -160,6 +160,22 @@ extern "C"
         return IFGF_OK;
     }
 
+    int ifgf_mult_noalloc(ifgf_operator* op, const double* w_dev, double* out_dev, void* stream)
+    {
+        if (!op)
+            return IFGF_EINVAL;
+        if (!op->on_device)
+            return fail(op, IFGF_ESTATE, "mult before a device init");
+        if (!w_dev || !out_dev)
+            return fail(op, IFGF_EINVAL, "mult needs device densities and a device output");
+        std::string e = device_mult_noalloc(op->dev, w_dev, out_dev, stream);
+        if (!e.empty())
+            return fail(op, e.rfind("allocation-free mult: ", 0) == 0 ? IFGF_EINVAL
+                                                                      : IFGF_EINTERNAL,
+                        e);
+        return IFGF_OK;
+    }
+
     int ifgf_mult_adjoint(ifgf_operator* op, const double* g, int64_t nrhs, double* out,
                           int flags, void* stream)
     {

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <type_traits>
 #include <utility>
@@ -1122,20 +1123,32 @@ struct PhaseTimer
     }
 };
 
+// The Morton-order densities and results of one single-column mult: the plan's own vectors,
+// or the allocation-free mult's fixed scratch (which also keeps the mult on one stream).
+struct Stage
+{
+    void* ws;
+    void* res;
+    bool side; // the side stream of the overlapped near and far fields may be used
+};
+inline Stage plan_stage(const DevicePlan& D) { return {D.wsorted, D.result, true}; }
+
 // everything of a mult between the gather of the densities and the scatter of the results:
-// it reads and writes only the plan's buffers, so one captured graph serves every call
-template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt)
+// with the plan's stage it reads and writes only the plan's buffers, so one captured graph
+// serves every call
+template <int KIND, typename R>
+void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
 {
     constexpr bool PH = KIND != KL;
     const KParams kp = kparams(D);
     const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
-    Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
-    Cx<R>* res = static_cast<Cx<R>*>(D.result);
+    Cx<R>* ws = static_cast<Cx<R>*>(sg.ws);
+    Cx<R>* res = static_cast<Cx<R>*>(sg.res);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
     // near and far fields: SidePipe (overlapped with the upward chain on the side stream when
     // the shape asks for it; phase timing runs everything in order on one stream)
-    const int overlap = (pt.on || !D.n_near_items) ? 0 : D.shape.overlap;
+    const int overlap = (pt.on || !D.n_near_items || !sg.side) ? 0 : D.shape.overlap;
     pt.begin(0);
     k_zero<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(res, D.ntgt);
     auto pipe = side_pipe(
@@ -1188,10 +1201,10 @@ template <int KIND, typename R> void run_core(DevicePlan& D, hipStream_t st, Pha
 // one mult: gather, the core (eagerly, or as the plan's captured graph), scatter
 template <int KIND, typename R>
 void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t st, bool timed,
-              int io, bool graph)
+              int io, bool graph, const Stage& sg)
 {
-    Cx<R>* ws = static_cast<Cx<R>*>(D.wsorted);
-    Cx<R>* res = static_cast<Cx<R>*>(D.result);
+    Cx<R>* ws = static_cast<Cx<R>*>(sg.ws);
+    Cx<R>* res = static_cast<Cx<R>*>(sg.res);
     PhaseTimer pt{timed, st, {}};
     const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
     const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
@@ -1211,7 +1224,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
             D.cap_stream = cs;
             PhaseTimer none{false, cs, {}};
             HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            run_core<KIND, R>(D, cs, none);
+            run_core<KIND, R>(D, cs, none, sg);
             hipGraph_t g = nullptr;
             HIP_OK(hipStreamEndCapture(cs, &g));
             hipGraphExec_t ex = nullptr;
@@ -1222,7 +1235,7 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
         HIP_OK(hipGraphLaunch(static_cast<hipGraphExec_t>(D.graph_exec), st));
     }
     else
-        run_core<KIND, R>(D, st, pt);
+        run_core<KIND, R>(D, st, pt, sg);
     pt.begin(0);
     k_scatter<R><<<dim3(unsigned((D.ntgt + 255) / 256)), dim3(256), 0, st>>>(
         res, D.tgt_perm, out_dev, D.ntgt, out_re, out_im);
@@ -1233,24 +1246,59 @@ void run_mult(DevicePlan& D, const double* w_dev, double* out_dev, hipStream_t s
 
 template <typename R>
 void dispatch(DevicePlan& D, const double* w, double* o, hipStream_t st, bool timed, int io,
-              bool graph)
+              bool graph, const Stage& sg)
 {
     switch (D.kind)
     {
     case KIND_HELMHOLTZ:
-        return run_mult<KH, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KH, R>(D, w, o, st, timed, io, graph, sg);
     case KIND_MODHELMHOLTZ:
-        return run_mult<KM, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KM, R>(D, w, o, st, timed, io, graph, sg);
     case KIND_LAPLACE:
-        return run_mult<KL, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KL, R>(D, w, o, st, timed, io, graph, sg);
     case KIND_DOUBLELAYER:
-        return run_mult<KD, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KD, R>(D, w, o, st, timed, io, graph, sg);
     case KIND_COMBINED:
-        return run_mult<KC, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KC, R>(D, w, o, st, timed, io, graph, sg);
     case KIND_GRAD:
-        return run_mult<KG, R>(D, w, o, st, timed, io, graph);
+        return run_mult<KG, R>(D, w, o, st, timed, io, graph, sg);
     }
     throw std::runtime_error("unknown kernel kind");
+}
+
+// The fixed device scratch of the allocation-free mult (device_mult_noalloc): the Morton-order
+// densities and results of one column.  Every GPU the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch), never by its name.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+// the scratch's device address on the current device, looked up once per device
+void* noalloc_scratch(std::string& err)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_SCRATCH_DEVICES)
+    {
+        err = "allocation-free mult: no current HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+    {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+        {
+            err = std::string("allocation-free mult: hipGetSymbolAddress(scratch) failed: ") +
+                  hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 } // namespace
@@ -1299,6 +1347,13 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
     try
     {
         device_free(D);
+        HIP_OK(hipGetDevice(&D.device));
+        {
+            // the allocation-free mult's scratch address on this device (once per device)
+            std::string serr;
+            if (!noalloc_scratch(serr))
+                throw std::runtime_error(serr);
+        }
         D.prec = precision;
         D.kind = P.K.kind;
         D.k_re = P.K.k.real();
@@ -1837,14 +1892,53 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 odev = obuf;
             }
             if (D.prec == PREC_FP64)
-                dispatch<double>(D, wdev, odev, st, timed, io, graph);
+                dispatch<double>(D, wdev, odev, st, timed, io, graph, plan_stage(D));
             else
-                dispatch<float>(D, wdev, odev, st, timed, io, graph);
+                dispatch<float>(D, wdev, odev, st, timed, io, graph, plan_stage(D));
             if (!device_ptrs)
                 HIP_OK(hipMemcpyAsync(orr, obuf, 16 * size_t(D.ntgt), hipMemcpyDeviceToHost, st));
         }
         if (!device_ptrs)
             HIP_OK(hipStreamSynchronize(st));
+    }
+    catch (const std::exception& e)
+    {
+        return e.what();
+    }
+    return "";
+}
+
+std::string device_mult_noalloc(DevicePlan& D, const double* w_dev, double* out_dev,
+                                void* stream)
+{
+    try
+    {
+        int dev = -1;
+        HIP_OK(hipGetDevice(&dev));
+        if (dev != D.device)
+            return "allocation-free mult: the current HIP device is not the operator's (init "
+                   "ran on device " + std::to_string(D.device) + ")";
+        const size_t cs = D.prec == PREC_FP64 ? 16 : 8;
+        const size_t ws_bytes = (cs * size_t(D.nsrc) + 255) / 256 * 256;
+        const size_t need = ws_bytes + cs * size_t(D.ntgt);
+        if (need > NOALLOC_SCRATCH_BYTES)
+            return "allocation-free mult: " + std::to_string(D.nsrc) + " sources and " +
+                   std::to_string(D.ntgt) + " targets need " + std::to_string(need) +
+                   " bytes of scratch, more than its " +
+                   std::to_string(NOALLOC_SCRATCH_BYTES) + " (n_src + n_tgt <= " +
+                   std::to_string(NOALLOC_SCRATCH_BYTES / cs - 32) + " in this precision)";
+        std::string err;
+        unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(err));
+        if (!base)
+            return err;
+        // the results are zeroed by the mult's first kernel and the densities fully written by
+        // the gather, so what an earlier call left in the scratch is never read
+        const Stage sg{base, base + ws_bytes, false};
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (D.prec == PREC_FP64)
+            dispatch<double>(D, w_dev, out_dev, st, false, 0, false, sg);
+        else
+            dispatch<float>(D, w_dev, out_dev, st, false, 0, false, sg);
     }
     catch (const std::exception& e)
     {

@@ -73,8 +73,14 @@ struct LaunchShape
 };
 constexpr int UP_K_CHOICES[4] = {2, 4, 8, 16};
 
+// bytes of the fixed device scratch of the allocation-free mult (device_mult_noalloc): the
+// Morton-order densities and results of one column (64 MiB: n_src + n_tgt up to 8,388,576
+// points in the mixed form, 4,194,272 in FP64)
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+
 struct DevicePlan
 {
+    int device = -1; // HIP device of the upload
     int prec = PREC_MIXED;
     int kind = 0;
     double k_re = 0, k_im = 0; // e^{-k d}
@@ -204,6 +210,12 @@ enum
 };
 std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, bool device_ptrs,
                         void* stream, int io = 0, bool graph = false);
+// One column of device densities into device results with no device allocation: the staging
+// vectors live in the library's fixed per-device scratch (NOALLOC_SCRATCH_BYTES; larger
+// operators are refused), everything on `stream`, no phase timing, no graph.  Calls that
+// overlap on one GPU race on the scratch.  The current device must be the upload's.
+std::string device_mult_noalloc(DevicePlan& D, const double* w_dev, double* out_dev,
+                                void* stream);
 // Apply a launch shape (validated; an unknown choice is an error).  Drops a captured graph.
 std::string device_set_shape(DevicePlan& D, const LaunchShape& s);
 // Drop the captured graph of the single-column mult (after a change of its kernel arguments).

@@ -76,6 +76,19 @@ int ifgf_init(ifgf_operator* op, const double* srcs, int64_t n_src, const double
 int ifgf_mult(ifgf_operator* op, const double* w, int64_t nrhs, double* out, int flags,
               void* stream);
 
+/* Allocation-free mult: one column of device densities w_dev (complex128, N) into device
+ * results out_dev (complex128, M), enqueued on stream (NULL = default stream); allocates no
+ * device memory, creates no stream, event or graph, and runs every kernel on that one stream.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The Morton-order
+ * densities and results live in a fixed 64 MiB device array of the library (one per GPU, held
+ * until the process ends, its address looked up at ifgf_init): operators with
+ * n_src + n_tgt > 8,388,576 (mixed) or 4,194,272 (FP64) get IFGF_EINVAL.  The results are
+ * those of ifgf_mult with IFGF_DEVICE_POINTERS, bit for bit.  Calls that overlap on one GPU
+ * (concurrent streams or threads) race on the array.  The current device must be the one of
+ * ifgf_init. */
+int ifgf_mult_noalloc(ifgf_operator* op, const double* w_dev, double* out_dev, void* stream);
+
 /* The adjoint product A^H g of mult (out: N values per column, g: M).  No counterpart in the
  * reference, which has no adjoint; it serves the autograd wrapper's backward (the gradient
  * with respect to the densities).  For the potentials G(x - y) = G(y - x) and for the

@@ -38,6 +38,7 @@ def test_version_and_errors_without_device():
     w = np.zeros(8, complex)
     assert L.ifgf_mult(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
     assert L.ifgf_mult_adjoint(h, w.ctypes.data, 1, w.ctypes.data, 0, None) == _lib.IFGF_ESTATE
+    assert L.ifgf_mult_noalloc(h, w.ctypes.data, w.ctypes.data, None) == _lib.IFGF_ESTATE
     assert L.ifgf_set_dx(h, 1) == _lib.IFGF_EINVAL  # not a gradient operator
     shape = np.zeros(6, np.int32)
     assert L.ifgf_autotune(h, 1, 1, shape.ctypes.data) == _lib.IFGF_ESTATE
