@@ -19,6 +19,7 @@
 // Compiled without -ffast-math and with -ffp-contract=off.
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <complex>
@@ -1380,8 +1381,9 @@ struct Operator
     }
 
     // the recursive phase restricted to every stride-th box of the first recursive level;
-    // returns the fraction of source points in the sampled subtrees (bounded CPU timing)
-    double mult_sample(const cplx* weights, int stride) const
+    // returns the fraction of source points in the sampled subtrees (bounded CPU timing) and,
+    // in thread_seconds, the sum of the subtrees' durations (each runs on one thread)
+    double mult_sample(const cplx* weights, int stride, double* thread_seconds = nullptr) const
     {
         std::vector<cplx> w(nsrc);
         for (int64_t i = 0; i < nsrc; i++)
@@ -1408,14 +1410,25 @@ struct Operator
         std::vector<std::vector<cplx>> thread_res(std::max(1, nthreads));
         for (auto& r : thread_res)
             r.assign(ntgt, cplx(0));
+        std::vector<double> dur(ids.size(), 0.0);
         parallel_for(int64_t(ids.size()), nthreads,
                      [&](int64_t q, int th)
                      {
+                         const auto t0 = std::chrono::steady_clock::now();
                          const Node& nd = Lr[ids[q]];
                          std::vector<cplx> up(idata[nd.parent].values.size(), cplx(0));
                          recursive_mult(min_rec, int(ids[q]), w.data(), thread_res[th].data(),
                                         &idata[nd.parent], &up);
+                         dur[q] = std::chrono::duration<double>(
+                                      std::chrono::steady_clock::now() - t0)
+                                      .count();
                      });
+        if (thread_seconds)
+        {
+            *thread_seconds = 0.0;
+            for (double d : dur)
+                *thread_seconds += d;
+        }
         return double(pts) / double(nsrc);
     }
 
@@ -1661,11 +1674,12 @@ extern "C"
             h->op->cutoff = nullptr;
     }
 
-    double orc_mult_sample(void* hv, const double* w, int stride, int nthreads)
+    double orc_mult_sample(void* hv, const double* w, int stride, int nthreads,
+                           double* thread_seconds)
     {
         auto* h = static_cast<OrcHandle*>(hv);
         h->op->nthreads = nthreads;
-        return h->op->mult_sample(reinterpret_cast<const cplx*>(w), stride);
+        return h->op->mult_sample(reinterpret_cast<const cplx*>(w), stride, thread_seconds);
     }
 
     // util.hpp:214-234 on an interleaved complex array of rows = n0 n1 n2 values

@@ -45,7 +45,7 @@ def lib(variant=""):
         L.orc_base_order.argtypes = [P, P]
         L.orc_set_cutoff.argtypes = [P, D]
         L.orc_mult_sample.restype = D
-        L.orc_mult_sample.argtypes = [P, P, I, I]
+        L.orc_mult_sample.argtypes = [P, P, I, I, P]
         L.orc_levels.argtypes = [P, I]
         L.orc_num_boxes.restype = I64
         L.orc_num_boxes.argtypes = [P, I, I]
@@ -140,9 +140,13 @@ class Oracle:
 
     def mult_sample(self, w, stride, nthreads=None):
         """Recursive phase on every stride-th first-recursive-level subtree; returns the
-        fraction of source points covered (for a bounded CPU timing)."""
+        fraction of source points covered and the summed duration of the subtrees, each of
+        which runs on one thread (thread-seconds; for a bounded CPU timing)."""
         w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
-        return self._L.orc_mult_sample(self.h, _p(w), int(stride), nthreads or default_threads())
+        ts = np.zeros(1)
+        frac = self._L.orc_mult_sample(self.h, _p(w), int(stride), nthreads or default_threads(),
+                                       _p(ts))
+        return frac, float(ts[0])
 
     def base_order(self):
         o = np.zeros(3, dtype=np.int32)
