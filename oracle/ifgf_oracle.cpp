@@ -383,30 +383,37 @@ static void chebtransform3(const cplx* src, cplx* dst, const int ns[3])
     chebtransform_rec(src, dst, ns, 3);
 }
 
-// chebinterp.hpp:252-346: tensor Clenshaw, outermost recursion over the last axis.
-static cplx clenshaw_rec(const double* x, const cplx* c, const int* ns, int dim)
+// chebinterp.hpp:252-346: tensor Clenshaw, outermost recursion over the last axis (the
+// coefficients of an axis are the Clenshaw sums over the axes before it), the recursion
+// written out for one, two and three axes; no halving of c_0
+static inline cplx clen_d1(double x, const cplx* c, int n)
 {
-    const int Nd = ns[dim - 1];
-    int stride = 1;
-    for (int i = 0; i < dim - 1; i++)
-        stride *= ns[i];
-    auto coef = [&](int j) -> cplx
-    { return dim == 1 ? c[j] : clenshaw_rec(x, c + size_t(j) * stride, ns, dim - 1); };
-    const double xd = x[dim - 1];
-    if (Nd <= 2)
+    if (n == 1)
+        return c[0];
+    if (n == 2)
+        return x * c[1] + c[0];
+    cplx b2 = c[n - 1];
+    cplx b1 = 2. * x * c[n - 1] + c[n - 2];
+    for (int j = n - 3; j > 0; j--)
     {
-        if (Nd == 1)
-            return coef(0);
+        cplx tmp = c[j] + (2. * (b1 * x) - b2);
+        b2 = b1;
+        b1 = tmp;
+    }
+    return (b1 * x - b2) + c[0];
+}
+
+static inline cplx clen_d2(const double* x, const cplx* c, int n0, int n1)
+{
+    auto coef = [&](int j) { return clen_d1(x[0], c + size_t(j) * n0, n0); };
+    const double xd = x[1];
+    if (n1 == 1)
+        return coef(0);
+    if (n1 == 2)
         return xd * coef(1) + coef(0);
-    }
-    cplx b2 = coef(Nd - 1);
-    cplx b1 = 2. * b2 * xd + coef(Nd - 2);
-    if (dim == 1)
-    {
-        b1 = 2. * xd * c[Nd - 1] + c[Nd - 2];
-        b2 = c[Nd - 1];
-    }
-    for (int j = Nd - 3; j > 0; j--)
+    cplx b2 = coef(n1 - 1);
+    cplx b1 = 2. * b2 * xd + coef(n1 - 2);
+    for (int j = n1 - 3; j > 0; j--)
     {
         cplx tmp = coef(j) + (2. * (b1 * xd) - b2);
         b2 = b1;
@@ -417,7 +424,23 @@ static cplx clenshaw_rec(const double* x, const cplx* c, const int* ns, int dim)
 
 static cplx clenshaw3(const double xi[3], const cplx* c, const int ns[3])
 {
-    return clenshaw_rec(xi, c, ns, 3);
+    const int n0 = ns[0], n1 = ns[1], n2 = ns[2];
+    const size_t stride = size_t(n0) * n1;
+    auto coef = [&](int j) { return clen_d2(xi, c + j * stride, n0, n1); };
+    const double xd = xi[2];
+    if (n2 == 1)
+        return coef(0);
+    if (n2 == 2)
+        return xd * coef(1) + coef(0);
+    cplx b2 = coef(n2 - 1);
+    cplx b1 = 2. * b2 * xd + coef(n2 - 2);
+    for (int j = n2 - 3; j > 0; j--)
+    {
+        cplx tmp = coef(j) + (2. * (b1 * xd) - b2);
+        b2 = b1;
+        b1 = tmp;
+    }
+    return (b1 * xd - b2) + coef(0);
 }
 
 // chebinterp.cpp:145-229 evaluated pointwise: the values of a tensor interpolant at the
