@@ -845,9 +845,11 @@ void launch_transform_b(DevicePlan& D, hipStream_t st, int64_t c0, int64_t c1, c
                                                        D.shape.xcd_log2);
 }
 
+// everything of a batched mult between the gather of the densities and the scatter of the
+// results: it reads and writes only the batch buffers, so one captured graph serves every call
+// with these buffers
 template <int KIND, typename R, int RB, int O0, int O1, int O2>
-void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, hipStream_t st,
-                 int io)
+void run_batched_core(DevicePlan& D, hipStream_t st)
 {
     constexpr bool PH = KIND != KL;
     constexpr bool P6 = O1 == 3;
@@ -858,12 +860,7 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
     Cx<R>* res = static_cast<Cx<R>*>(D.b_res);
     Cx<R>* V = static_cast<Cx<R>*>(D.b_V);
     Cx<R>* F = static_cast<Cx<R>*>(D.b_F);
-    const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
-    const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
-    const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
-    const int64_t ns = D.nsrc * RB, nt = D.ntgt * RB;
-    k_gather_b<R><<<dim3(unsigned((ns + 255) / 256)), dim3(256), 0, st>>>(
-        w_dev, D.src_perm, ws, D.nsrc, RB, ncol, in_im);
+    const int64_t nt = D.ntgt * RB;
     k_zero_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(res, nt);
     // near and far fields as in the single-column mult (SidePipe)
     const int overlap = D.n_near_items ? D.shape.overlap : 0;
@@ -949,39 +946,82 @@ void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, 
         pipe.ready(l);
     }
     pipe.finish();
+}
+
+// one batched mult: gather, the core (eagerly, or as the plan's captured batched graph), scatter
+template <int KIND, typename R, int RB, int O0, int O1, int O2>
+void run_batched(DevicePlan& D, const double* w_dev, double* out_dev, int ncol, hipStream_t st,
+                 int io, bool graph)
+{
+    Cx<R>* ws = static_cast<Cx<R>*>(D.b_ws);
+    Cx<R>* res = static_cast<Cx<R>*>(D.b_res);
+    const double in_im = (io & IO_CONJ_IN) ? -1.0 : 1.0;
+    const double out_re = (io & IO_NEGATE_OUT) ? -1.0 : 1.0;
+    const double out_im = out_re * ((io & IO_CONJ_OUT) ? -1.0 : 1.0);
+    const int64_t ns = D.nsrc * RB, nt = D.ntgt * RB;
+    k_gather_b<R><<<dim3(unsigned((ns + 255) / 256)), dim3(256), 0, st>>>(
+        w_dev, D.src_perm, ws, D.nsrc, RB, ncol, in_im);
+    if (graph)
+    {
+        if (!D.b_graph_exec)
+        {
+            // captured on the plan's capture stream (the caller's may be the null stream,
+            // which cannot be captured); kernels and event waits only
+            if (!D.cap_stream)
+            {
+                hipStream_t cs;
+                HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+                D.cap_stream = cs;
+            }
+            hipStream_t cs = static_cast<hipStream_t>(D.cap_stream);
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            run_batched_core<KIND, R, RB, O0, O1, O2>(D, cs);
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamEndCapture(cs, &g));
+            hipGraphExec_t ex = nullptr;
+            HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+            D.b_graph = g;
+            D.b_graph_exec = ex;
+        }
+        HIP_OK(hipGraphLaunch(static_cast<hipGraphExec_t>(D.b_graph_exec), st));
+    }
+    else
+        run_batched_core<KIND, R, RB, O0, O1, O2>(D, st);
     k_scatter_b<R><<<dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st>>>(
         res, D.tgt_perm, out_dev, D.ntgt, RB, ncol, out_re, out_im);
     HIP_OK(hipGetLastError());
 }
 
 template <typename R, int RB, int O0, int O1, int O2>
-void run_kind(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t st, int io)
+void run_kind(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t st, int io,
+              bool graph)
 {
     switch (D.kind)
     {
     case KIND_HELMHOLTZ:
-        return run_batched<KH, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KH, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     case KIND_MODHELMHOLTZ:
-        return run_batched<KM, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KM, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     case KIND_LAPLACE:
-        return run_batched<KL, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KL, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     case KIND_DOUBLELAYER:
-        return run_batched<KD, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KD, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     case KIND_COMBINED:
-        return run_batched<KC, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KC, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     case KIND_GRAD:
-        return run_batched<KG, R, RB, O0, O1, O2>(D, w, o, ncol, st, io);
+        return run_batched<KG, R, RB, O0, O1, O2>(D, w, o, ncol, st, io, graph);
     }
     throw std::runtime_error("unknown kernel kind");
 }
 
 template <typename R, int RB>
-void run_order(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t st, int io)
+void run_order(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t st, int io,
+               bool graph)
 {
     if (order_class(D) == 1)
-        run_kind<R, RB, 2, 3, 3>(D, w, o, ncol, st, io);
+        run_kind<R, RB, 2, 3, 3>(D, w, o, ncol, st, io, graph);
     else
-        run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io);
+        run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io, graph);
 }
 } // namespace
 
@@ -992,7 +1032,7 @@ bool batched_supported(const DevicePlan& D)
 }
 
 void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, int ncol,
-                  void* stream, int io)
+                  void* stream, int io, bool graph)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (!batched_supported(D) || ncol < 1 || ncol > rb || rb != D.b_rb || (rb != 8 && rb != 32))
@@ -1000,16 +1040,16 @@ void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, i
     if (D.prec == PREC_FP64)
     {
         if (rb == 8)
-            run_order<double, 8>(D, w_dev, out_dev, ncol, st, io);
+            run_order<double, 8>(D, w_dev, out_dev, ncol, st, io, graph);
         else
-            run_order<double, 32>(D, w_dev, out_dev, ncol, st, io);
+            run_order<double, 32>(D, w_dev, out_dev, ncol, st, io, graph);
     }
     else
     {
         if (rb == 8)
-            run_order<float, 8>(D, w_dev, out_dev, ncol, st, io);
+            run_order<float, 8>(D, w_dev, out_dev, ncol, st, io, graph);
         else
-            run_order<float, 32>(D, w_dev, out_dev, ncol, st, io);
+            run_order<float, 32>(D, w_dev, out_dev, ncol, st, io, graph);
     }
 }
 } // namespace ifgf

@@ -858,6 +858,21 @@ struct UpEntD
 };
 template <typename R> using UpEnt = typename std::conditional<sizeof(R) == 4, UpEntF, UpEntD>::type;
 
+// The upward table of one (cell, octant) regrouped for evaluation (FP32): the parent nodes
+// whose positions fall in one child fine cone, in runs of at most UP_GRP nodes, so that one
+// thread evaluates the interpolant of that cone at all of them from one read of its
+// coefficients (96 B, six float4)
+constexpr int UP_GRP = 4;
+struct UpGrp
+{
+    int32_t cf;          // coarse cell of the child grid holding the fine cone
+    int32_t subd;        // sub-cone index (bits 0-7) | index d of the cone in its list << 8
+    uint32_t q01, q23;   // parent node indices, 16 bits each (0xffff: an empty place)
+    float x0[UP_GRP], x1[UP_GRP], x2[UP_GRP]; // local coordinates in the fine cone
+    float tr[UP_GRP], ti[UP_GRP];             // transfer factors
+};
+static_assert(sizeof(UpGrp) == 96, "six float4");
+
 // the kernel parameters with the wavenumber of box B (plans of several geometries carry one
 // per box)
 __device__ __forceinline__ KParams with_k(KParams kp, const BoxGeo& B)

@@ -350,142 +350,263 @@ __global__ __launch_bounds__(128) void k_up_cell(const WorkItem* __restrict__ it
     }
 }
 
-// upward pass with the per-level tables and the child coefficients staged in LDS (FP32,
-// compiled orders).  The entries of one (cell, octant) fall into few distinct child cones
-// (about 15 for 336 nodes at orders (6,8,7)/(3,5,4)), listed per (cell, octant) at upload
-// (UpEntF::d).  Per parent cone k of the item and octant c the workgroup copies the
-// coefficient blocks of those cones into LDS once, and every node reads its block from there
-// (k_up_cell loads a block from memory per node, and the loads bound it).  Lists longer than
-// DMAX read the rest from memory.  Per node the same evaluations and the same sum over the
-// children in the same order as k_up_cell.  Held to 5 waves per SIMD (no spills at 2 and 4
-// cones per item; DMAX = 24 keeps the LDS from limiting it further).
-template <int O0, int O1, int O2, int K, int QPT, int DMAX>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k_up_lds(
-                                                const WorkItem* __restrict__ items,
-                                                int64_t nitems,
-                                                const int32_t* __restrict__ glist,
-                                                const ConeDev* __restrict__ cones,
-                                                const BoxGeo* __restrict__ geo,
-                                                const int32_t* __restrict__ child,
-                                                const CellRef* __restrict__ ctab,
-                                                const UpEntF* __restrict__ ents,
-                                                const int64_t* __restrict__ part_ent,
-                                                const int64_t* __restrict__ part_cell,
-                                                const int32_t* __restrict__ doff,
-                                                const int32_t* __restrict__ dkey,
-                                                const Cx<float>* __restrict__ F,
-                                                Cx<float>* __restrict__ V, KParams kp)
+// Chebyshev values T_1 .. T_{n-1} at x (T_0 = 1 is implied)
+template <int N> __device__ __forceinline__ void cheb_t(float x, float (&t)[N])
+{
+    if constexpr (N > 1)
+        t[1] = x;
+#pragma unroll
+    for (int k = 2; k < N; k++)
+        t[k] = fmaf(2.0f * x, t[k - 1], -(k == 2 ? 1.0f : t[k - 2]));
+}
+
+// The interpolants of one fine cone (coefficients c, index i0 + O0 (i1 + O1 i2)) at G points
+// from their Chebyshev values: sum_{i0,i1,i2} c T_i0(x0) T_i1(x1) T_i2(x2), summed over i0,
+// then i1, then i2 (each coefficient read once for the G points)
+template <int O0, int O1, int O2, int G>
+__device__ __forceinline__ void eval_cone_g(const float2* __restrict__ c,
+                                            const float (&t0)[G][O0], const float (&t1)[G][O1],
+                                            const float (&t2)[G][O2], float2 (&out)[G])
+{
+#pragma unroll
+    for (int i2 = 0; i2 < O2; i2++)
+    {
+        float2 pl[G];
+#pragma unroll
+        for (int i1 = 0; i1 < O1; i1++)
+        {
+            float2 row[O0];
+#pragma unroll
+            for (int i0 = 0; i0 < O0; i0++)
+                row[i0] = c[i0 + O0 * (i1 + O1 * i2)];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+            {
+                float2 sm = row[0];
+#pragma unroll
+                for (int i0 = 1; i0 < O0; i0++)
+                {
+                    sm.x = fmaf(row[i0].x, t0[g][i0], sm.x);
+                    sm.y = fmaf(row[i0].y, t0[g][i0], sm.y);
+                }
+                if (i1 == 0)
+                    pl[g] = sm;
+                else
+                {
+                    pl[g].x = fmaf(sm.x, t1[g][i1], pl[g].x);
+                    pl[g].y = fmaf(sm.y, t1[g][i1], pl[g].y);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+        {
+            if (i2 == 0)
+                out[g] = pl[g];
+            else
+            {
+                out[g].x = fmaf(pl[g].x, t2[g][i2], out[g].x);
+                out[g].y = fmaf(pl[g].y, t2[g][i2], out[g].y);
+            }
+        }
+    }
+}
+
+// eval_cone_g with the coefficients in memory, read one axis-2 plane at a time (the loads of a
+// plane are not hoisted above the previous plane's arithmetic: fewer live registers)
+template <int O0, int O1, int O2, int G>
+__device__ __forceinline__ void eval_cone_g_mem(const float2* c,
+                                                const float (&t0)[G][O0], const float (&t1)[G][O1],
+                                                const float (&t2)[G][O2], float2 (&out)[G])
+{
+    constexpr int NP = O0 * O1;
+    constexpr int PB = (O2 % 2 == 0) ? 2 : 1; // planes per batch of loads
+    float2 cp2[PB * NP];
+#pragma unroll
+    for (int i2 = 0; i2 < O2; i2++)
+    {
+        if (i2 % PB == 0)
+        {
+#pragma unroll
+            for (int i = 0; i < PB * NP; i++)
+                cp2[i] = c[NP * i2 + i];
+        }
+        const float2* cp = cp2 + NP * (i2 % PB);
+        float2 pl[G];
+#pragma unroll
+        for (int i1 = 0; i1 < O1; i1++)
+        {
+#pragma unroll
+            for (int g = 0; g < G; g++)
+            {
+                float2 sm = cp[O0 * i1];
+#pragma unroll
+                for (int i0 = 1; i0 < O0; i0++)
+                {
+                    sm.x = fmaf(cp[O0 * i1 + i0].x, t0[g][i0], sm.x);
+                    sm.y = fmaf(cp[O0 * i1 + i0].y, t0[g][i0], sm.y);
+                }
+                if (i1 == 0)
+                    pl[g] = sm;
+                else
+                {
+                    pl[g].x = fmaf(sm.x, t1[g][i1], pl[g].x);
+                    pl[g].y = fmaf(sm.y, t1[g][i1], pl[g].y);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+        {
+            if (i2 == 0)
+                out[g] = pl[g];
+            else
+            {
+                out[g].x = fmaf(pl[g].x, t2[g][i2], out[g].x);
+                out[g].y = fmaf(pl[g].y, t2[g][i2], out[g].y);
+            }
+        }
+        // the next batch's address depends on this batch's sums: its loads follow them
+        if (i2 % PB == PB - 1)
+            asm volatile("" : "+v"(c) : "v"(out[0].x));
+    }
+}
+
+// Upward pass with the grouped tables (FP32, compiled orders; ifgfoperator.hpp:631-688,
+// 1639-1705).  A workgroup takes the parent cones of an item one at a time; all the cone's
+// (child, group) tasks run at once: a group is up to UP_GRP nodes falling in one child cone,
+// whose coefficients the thread reads once (from memory) for all of them, and writes each
+// node's value times its transfer factor to the node's partial sum of that child in LDS.  The
+// partial sums of every node are then added in child order, as the reference's.
+template <int O0, int O1, int O2, int NHI, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_up_grp(
+    const WorkItem* __restrict__ items, int64_t nitems, const int32_t* __restrict__ glist,
+    const ConeDev* __restrict__ cones, const BoxGeo* __restrict__ geo,
+    const int32_t* __restrict__ child, const CellRef* __restrict__ ctab,
+    const UpGrp* __restrict__ grp, const int32_t* __restrict__ goff,
+    const int64_t* __restrict__ part_cell, const int32_t* __restrict__ doff,
+    const int32_t* __restrict__ dkey, const Cx<float>* __restrict__ F,
+    Cx<float>* __restrict__ V, KParams kp)
 {
     constexpr int NC = O0 * O1 * O2;
-    static_assert(NC % 2 == 0, "blocks of whole float4");
-    constexpr int NV = NC / 2;
-    __shared__ int64_t toff[K][8]; // table offset of child c of cone k; -1: no child
-    __shared__ int32_t gs[K];
-    __shared__ float4 blk[DMAX * NV];
-    // slot of listed cone d in child c of cone k (-1: not active in the child), found for the
-    // whole item up front so that each staging step waits for one load only
-    __shared__ int32_t sslot[K][8][DMAX];
-    __shared__ int32_t dcnt[8];
+    static_assert(UP_GRP == 4, "records of four nodes");
+    constexpr int G = UP_GRP;
+    constexpr int DMAX = 32;
+    __shared__ int64_t toff[8];     // table offset of child c; -1: no child
+    __shared__ int32_t tend[8];     // task list: child c's groups end at tend[c]
+    __shared__ int32_t tg0[8];      // record = tg0[c] + task index
+    __shared__ int32_t sslot[8][DMAX]; // slot of listed cone d of child c (-1: inactive)
+    __shared__ float2 part[8 * NHI + 1]; // node q's contribution from child c (+ a dump slot)
     const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
         return;
     const WorkItem wi = items[it];
     const int tid = threadIdx.x;
-    if (tid < K * 8)
-    {
-        const int k = tid >> 3, c = tid & 7;
-        int64_t off = -1;
-        if (k < wi.count)
-        {
-            const int32_t g = glist[wi.first + k];
-            if (c == 0)
-                gs[k] = g;
-            const int32_t ch = child[8 * cones[g].box + c];
-            if (ch >= 0)
-                off = geo[ch].table_off;
-        }
-        toff[k][c] = off;
-    }
-    __syncthreads();
-    const int n = wi.count;
-    const int nhi = kp.n_hi;
-    const UpEntF* ebox = ents + part_ent[wi.pad] + int64_t(wi.box) * nhi * 8;
     const int64_t row = (part_cell[wi.pad] + wi.box) * 8;
-    const float4* F4 = reinterpret_cast<const float4*>(F);
-    if (tid < 8)
-        dcnt[tid] = min(doff[row + tid + 1] - doff[row + tid], DMAX);
-    __syncthreads();
-    for (int i = tid; i < n * 8 * DMAX; i += 128)
-    {
-        const int k = i / (8 * DMAX), c = (i / DMAX) & 7, d = i % DMAX;
-        const int64_t to = toff[k][c];
-        if (to < 0 || d >= dcnt[c])
-            continue;
-        const int kd = doff[row + c] + d;
-        const int32_t cf = dkey[2 * kd], sub = dkey[2 * kd + 1];
-        const CellRef cr = ctab[to + cf];
-        sslot[k][c][d] =
-            ((cr.mask >> sub) & 1u) ? cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u)) : -1;
-    }
-    __syncthreads();
 #pragma unroll 1
-    for (int k = 0; k < n; k++)
+    for (int k = 0; k < wi.count; k++)
     {
-        Cx<float> acc[QPT];
-#pragma unroll
-        for (int i = 0; i < QPT; i++)
-            acc[i] = {0, 0};
-#pragma unroll 1
-        for (int c = 0; c < 8; c++)
+        const int32_t g = glist[wi.first + k];
+        if (tid < 8)
         {
-            const int64_t to = toff[k][c];
-            if (to < 0) // the same for the whole workgroup
+            const int32_t ch = child[8 * cones[g].box + tid];
+            toff[tid] = ch >= 0 ? geo[ch].table_off : -1;
+        }
+        for (int i = tid; i < 8 * NHI; i += NT)
+            part[i] = make_float2(0.0f, 0.0f);
+        __syncthreads();
+        if (tid == 0)
+        {
+            int tot = 0;
+            for (int c = 0; c < 8; c++)
+            {
+                const int ng = toff[c] >= 0 ? goff[row + c + 1] - goff[row + c] : 0;
+                tg0[c] = goff[row + c] - tot;
+                tot += ng;
+                tend[c] = tot;
+            }
+        }
+        for (int i = tid; i < 8 * DMAX; i += NT)
+        {
+            const int c = i / DMAX, d = i - c * DMAX;
+            int32_t sl = -1;
+            if (toff[c] >= 0 && d < doff[row + c + 1] - doff[row + c])
+            {
+                const int kd = doff[row + c] + d;
+                const int32_t cf = dkey[2 * kd], sub = dkey[2 * kd + 1];
+                const CellRef cr = ctab[toff[c] + cf];
+                if ((cr.mask >> sub) & 1u)
+                    sl = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
+            }
+            sslot[c][d] = sl;
+        }
+        __syncthreads();
+        const int tot = tend[7];
+        for (int t = tid; t < tot; t += NT)
+        {
+            int c = 0;
+            while (t >= tend[c])
+                c++;
+            // the record as six 16-byte loads: header, x0, x1, x2, tr, ti
+            const float4* ep = reinterpret_cast<const float4*>(grp + (tg0[c] + t));
+            const float4 hd = ep[0], ex0 = ep[1], ex1 = ep[2], ex2 = ep[3];
+            const int32_t cf = __float_as_int(hd.x), subd = __float_as_int(hd.y);
+            const int d = subd >> 8;
+            int slot;
+            if (d < DMAX)
+                slot = sslot[c][d];
+            else
+            {
+                const int sub = subd & 255;
+                const CellRef cr = ctab[toff[c] + cf];
+                slot = ((cr.mask >> sub) & 1u) ? cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u))
+                                               : -1;
+            }
+            if (slot < 0)
                 continue;
-            const int nd = dcnt[c];
-            for (int i = tid; i < nd * NV; i += 128)
-            {
-                const int d = i / NV, j = i - d * NV;
-                const int slot = sslot[k][c][d];
-                if (slot >= 0)
-                    blk[i] = F4[int64_t(slot) * NV + j];
-            }
-            __syncthreads();
+            const float xs0[G] = {ex0.x, ex0.y, ex0.z, ex0.w};
+            const float xs1[G] = {ex1.x, ex1.y, ex1.z, ex1.w};
+            const float xs2[G] = {ex2.x, ex2.y, ex2.z, ex2.w};
+            float t0[G][O0], t1[G][O1], t2[G][O2];
 #pragma unroll
-            for (int i = 0; i < QPT; i++)
+            for (int gg = 0; gg < G; gg++)
             {
-                const int q = tid + 128 * i;
-                if (q >= nhi)
-                    continue;
-                const UpEntF E = ebox[int64_t(q) * 8 + c];
-                if (E.cf < 0)
-                    continue;
-                Cx<float> val;
-                if (E.d < DMAX)
-                {
-                    if (sslot[k][c][E.d] < 0)
-                        continue;
-                    val = fine_eval_chunked<O0, O1, O2, float>(
-                        reinterpret_cast<const Cx<float>*>(blk), E.d, E.xi0, E.xi1, E.xi2);
-                }
-                else
-                {
-                    const CellRef cr = ctab[to + E.cf];
-                    if (!((cr.mask >> E.sub) & 1u))
-                        continue;
-                    const int slot = cr.fine_base + __popc(cr.mask & ((1u << E.sub) - 1u));
-                    val = fine_eval_chunked<O0, O1, O2, float>(F, slot, E.xi0, E.xi1, E.xi2);
-                }
-                acc[i] = cadd(acc[i], cmul(val, Cx<float>{E.tr, E.ti}));
+                cheb_t<O0>(xs0[gg], t0[gg]);
+                cheb_t<O1>(xs1[gg], t1[gg]);
+                cheb_t<O2>(xs2[gg], t2[gg]);
             }
-            __syncthreads();
+            float2 val[G];
+            eval_cone_g_mem<O0, O1, O2, G>(
+                reinterpret_cast<const float2*>(F + int64_t(slot) * NC), t0, t1, t2, val);
+            const float4 etr = ep[4], eti = ep[5];
+            const float trs[G] = {etr.x, etr.y, etr.z, etr.w};
+            const float tis[G] = {eti.x, eti.y, eti.z, eti.w};
+            const uint32_t qq[2] = {uint32_t(__float_as_int(hd.z)), uint32_t(__float_as_int(hd.w))};
+#pragma unroll
+            for (int gg = 0; gg < G; gg++)
+            {
+                // an empty place writes to the dump slot (no branch: the evaluation stays one
+                // straight sequence for the G points)
+                const int q = int((qq[gg >> 1] >> (16 * (gg & 1))) & 0xffffu);
+                const int at = q == 0xffff ? 8 * NHI : c * NHI + q;
+                const float tr = trs[gg], ti = tis[gg];
+                part[at] = make_float2(val[gg].x * tr - val[gg].y * ti,
+                                       val[gg].x * ti + val[gg].y * tr);
+            }
         }
-#pragma unroll
-        for (int i = 0; i < QPT; i++)
+        __syncthreads();
+        for (int q = tid; q < NHI; q += NT)
         {
-            const int q = tid + 128 * i;
-            if (q < nhi)
-                V[int64_t(gs[k]) * nhi + q] = acc[i];
+            float2 a = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+            {
+                const float2 p = part[c * NHI + q];
+                a = make_float2(a.x + p.x, a.y + p.y);
+            }
+            V[int64_t(g) * NHI + q] = Cx<float>{a.x, a.y};
         }
+        __syncthreads(); // before the next cone replaces toff and part
     }
 }
 
@@ -786,7 +907,31 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
             x1 = R(L.xi[1]);
             x2 = R(L.xi[2]);
         }
-        const Cx<R> val = fine_eval<O0, O1, O2, R>(F, fine, kp, x0, x1, x2);
+        Cx<R> val;
+        if constexpr (FAST && O0 > 0)
+        {
+            // FP32, compiled orders: the interpolant from its Chebyshev values, the coefficients
+            // read one row at a time
+            float t0[1][O0], t1[1][O1], t2[1][O2];
+            cheb_t<O0>(x0, t0[0]);
+            cheb_t<O1>(x1, t1[0]);
+            cheb_t<O2>(x2, t2[0]);
+            float2 v[1];
+            constexpr int NC = O0 * O1 * O2;
+            float2 cc[NC];
+            const float4* p4 = reinterpret_cast<const float4*>(F + int64_t(fine) * NC);
+#pragma unroll
+            for (int i = 0; i < NC / 2; i++)
+            {
+                const float4 q = p4[i];
+                cc[2 * i] = make_float2(q.x, q.y);
+                cc[2 * i + 1] = make_float2(q.z, q.w);
+            }
+            eval_cone_g<O0, O1, O2, 1>(cc, t0, t1, t2, v);
+            val = {v[0].x, v[0].y};
+        }
+        else
+            val = fine_eval<O0, O1, O2, R>(F, fine, kp, x0, x1, x2);
         Cx<R> cf;
         if constexpr (!PH)
             cf = {R(INV_4PI_D * rinv), R(0)};
@@ -860,29 +1005,25 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     if (up)
     {
         const int64_t i0 = D.up_item_off[up_level], i1 = D.up_item_off[up_level + 1];
-        if (i1 > i0)
+        if constexpr (sizeof(R) == 4 && O0 > 0)
+        {
+            // FP32, compiled orders: the grouped tables, the cones of an item one at a time
+            if (i1 > i0)
+                k_up_grp<O0, O1, O2, H0 * H1 * H2, 128>
+                    <<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
+                        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones,
+                        D.geo, D.child, D.ctab, static_cast<const UpGrp*>(D.up_grp), D.up_goff,
+                        D.up_part_cell, D.up_doff, D.up_dkey, F, V, kp);
+        }
+        else if (i1 > i0)
         {
 #define IFGF_UP_CELL(K)                                                                         \
-    if constexpr (sizeof(R) == 4 && O0 * O1 * O2 > 18)                                          \
-        k_up_lds<O0, O1, O2, K, (H0 * H1 * H2 + 127) / 128, 24>                                 \
-            <<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                                    \
-                D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,   \
-                D.child, D.ctab, static_cast<const UpEntF*>(D.up_entries), D.up_part_ent,       \
-                D.up_part_cell, D.up_doff, D.up_dkey, F, V, kp);                                \
-    else                                                                                        \
-        k_up_cell<R, O0, O1, O2, K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(            \
-            D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo,       \
-            D.child, D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V,  \
-            kp)
+    k_up_cell<R, O0, O1, O2, K><<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(                \
+        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones, D.geo, D.child,  \
+        D.ctab, static_cast<const UpEnt<R>*>(D.up_entries), D.up_part_ent, F, V, kp)
             switch (D.shape.up_k)
             {
-            case 2: // compiled for the LDS kernel only; the others take items of 2 as of 4
-                if constexpr (sizeof(R) == 4 && O0 * O1 * O2 > 18)
-                {
-                    IFGF_UP_CELL(2);
-                    break;
-                }
-                [[fallthrough]];
+            case 2: // items of at most 2 cones run as items of 4
             case 4:
                 IFGF_UP_CELL(4);
                 break;
@@ -1316,7 +1457,7 @@ void device_free(DevicePlan& D)
     void* ptrs[] = {D.src_pts, D.tgt_pts, D.normals, D.src_perm, D.tgt_perm, D.src_begin,
                     D.src_end, D.child, D.geo, D.cones, D.ctab, D.ho_nodes, D.trig_th,
                     D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_dkey,
-                    D.up_doff, D.up_part_cell, D.up_items_k[0],
+                    D.up_doff, D.up_part_cell, D.up_goff, D.up_grp, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
@@ -1328,6 +1469,10 @@ void device_free(DevicePlan& D)
         (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.graph_exec));
     if (D.graph)
         (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
+    if (D.b_graph_exec)
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.b_graph_exec));
+    if (D.b_graph)
+        (void)hipGraphDestroy(static_cast<hipGraph_t>(D.b_graph));
     if (D.cap_stream)
         (void)hipStreamDestroy(static_cast<hipStream_t>(D.cap_stream));
     if (D.side_stream)
@@ -1554,6 +1699,9 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
             std::vector<UpEntD> ed;
             std::vector<int64_t> part_ent, part_cell;
             std::vector<int32_t> dkey, doff;
+            std::vector<UpGrp> grp;
+            std::vector<int64_t> goff;
+            std::vector<int32_t> gcnt, gnode;
             for (const UpLevel* U : parts)
             {
                 part_ent.push_back(int64_t(precision == PREC_FP64 ? ed.size() : ef.size()));
@@ -1599,11 +1747,63 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                             E.d = d;
                             last = d;
                         }
+                        // the nodes regrouped by child cone (in list order, nodes ascending
+                        // within a cone), runs of UP_GRP
+                        const int nk = int(dkey.size() / 2) - k0;
+                        goff.push_back(int64_t(grp.size()));
+                        gcnt.assign(size_t(nk) + 1, 0);
+                        for (int q = 0; q < D.n_hi; q++)
+                        {
+                            const UpEntF& E = ef[e0 + size_t((cell * D.n_hi + q) * 8 + c)];
+                            if (E.cf >= 0)
+                                gcnt[size_t(E.d) + 1]++;
+                        }
+                        for (int i = 0; i < nk; i++)
+                            gcnt[i + 1] += gcnt[i];
+                        gnode.assign(size_t(gcnt[nk]), 0);
+                        for (int q = 0; q < D.n_hi; q++)
+                        {
+                            const UpEntF& E = ef[e0 + size_t((cell * D.n_hi + q) * 8 + c)];
+                            if (E.cf >= 0)
+                                gnode[size_t(gcnt[E.d]++)] = q;
+                        }
+                        for (size_t i0 = 0; i0 < gnode.size();)
+                        {
+                            const UpEntF& E0 = ef[e0 + size_t((cell * D.n_hi + gnode[i0]) * 8 + c)];
+                            UpGrp G{};
+                            G.cf = E0.cf;
+                            G.subd = E0.sub | (E0.d << 8);
+                            uint32_t qs[UP_GRP];
+                            size_t i = i0;
+                            for (int g = 0; g < UP_GRP; g++)
+                            {
+                                qs[g] = 0xffffu;
+                                if (i < gnode.size() &&
+                                    ef[e0 + size_t((cell * D.n_hi + gnode[i]) * 8 + c)].d == E0.d)
+                                {
+                                    const UpEntF& E = ef[e0 + size_t((cell * D.n_hi + gnode[i]) * 8 + c)];
+                                    qs[g] = uint32_t(gnode[i]);
+                                    G.x0[g] = E.xi0;
+                                    G.x1[g] = E.xi1;
+                                    G.x2[g] = E.xi2;
+                                    G.tr[g] = E.tr;
+                                    G.ti[g] = E.ti;
+                                    i++;
+                                }
+                            }
+                            G.q01 = qs[0] | (qs[1] << 16);
+                            G.q23 = qs[2] | (qs[3] << 16);
+                            grp.push_back(G);
+                            i0 = i;
+                        }
                     }
             }
             if (dkey.size() / 2 > size_t(INT32_MAX))
                 throw std::runtime_error("upward tables: too many distinct child cones");
+            if (grp.size() > size_t(INT32_MAX))
+                throw std::runtime_error("upward tables: too many node groups");
             doff.push_back(int32_t(dkey.size() / 2));
+            goff.push_back(int64_t(grp.size()));
             D.n_up_parts = int64_t(parts.size());
             D.up_part_ent = dupload(part_ent, tot);
             if (precision != PREC_FP64)
@@ -1611,6 +1811,9 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                 D.up_part_cell = dupload(part_cell, tot);
                 D.up_doff = dupload(doff, tot);
                 D.up_dkey = dupload(dkey, tot);
+                std::vector<int32_t> goff32(goff.begin(), goff.end());
+                D.up_goff = dupload(goff32, tot);
+                D.up_grp = dupload(grp, tot);
             }
             if (precision == PREC_FP64)
                 D.up_entries = dupload(ed, tot);
@@ -1642,8 +1845,23 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                     (P.S.leaf[g] ? leaf : (box_part[g] >= 0 ? glist : up)).push_back(int32_t(c));
             auto part = [&](int32_t c) { return box_part[P.cone_box[c]]; };
             auto cu = [&](int32_t c) { return parts[part(c)]->cellmap[P.cone_cell[c]]; };
+            // within a cell, cones of boxes with the same children next to one another: the
+            // cones of an item then have their children in the same octants
+            auto cmask = [&](int32_t c)
+            {
+                int m = 0;
+                for (int o = 0; o < 8; o++)
+                    m |= (P.S.child[8 * int64_t(P.cone_box[c]) + o] >= 0 ? 1 : 0) << o;
+                return m;
+            };
             std::stable_sort(glist.begin() + g0, glist.end(), [&](int32_t a, int32_t b)
-                             { return part(a) != part(b) ? part(a) < part(b) : cu(a) < cu(b); });
+                             {
+                                 if (part(a) != part(b))
+                                     return part(a) < part(b);
+                                 if (cu(a) != cu(b))
+                                     return cu(a) < cu(b);
+                                 return cmask(a) < cmask(b);
+                             });
             for (int kc = 0; kc < 4; kc++)
                 for (int64_t i = g0; i < int64_t(glist.size());)
                 {
@@ -1780,6 +1998,12 @@ static bool ensure_batch(DevicePlan& D, int rb, bool host_io)
 {
     if (D.b_rb == rb && (!host_io || D.b_in))
         return true;
+    // a captured batched graph holds the old buffers
+    if (D.b_graph_exec)
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.b_graph_exec));
+    if (D.b_graph)
+        (void)hipGraphDestroy(static_cast<hipGraph_t>(D.b_graph));
+    D.b_graph_exec = D.b_graph = nullptr;
     for (void* p : {D.b_ws, D.b_res, D.b_V, D.b_F, static_cast<void*>(D.b_in),
                     static_cast<void*>(D.b_out)})
         if (p)
@@ -1856,12 +2080,12 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                     double* oc = static_cast<double*>(out) + 2 * c * D.ntgt;
                     if (device_ptrs)
                     {
-                        mult_batched(D, wc, oc, rb, ncol, st, io);
+                        mult_batched(D, wc, oc, rb, ncol, st, io, graph && !timed);
                         continue;
                     }
                     HIP_OK(hipMemcpyAsync(D.b_in, wc, 16 * size_t(ncol) * size_t(D.nsrc),
                                           hipMemcpyHostToDevice, st));
-                    mult_batched(D, D.b_in, D.b_out, rb, ncol, st, io);
+                    mult_batched(D, D.b_in, D.b_out, rb, ncol, st, io, graph && !timed);
                     HIP_OK(hipMemcpyAsync(oc, D.b_out, 16 * size_t(ncol) * size_t(D.ntgt),
                                           hipMemcpyDeviceToHost, st));
                 }
@@ -1991,6 +2215,12 @@ void device_drop_graph(DevicePlan& D)
     if (D.graph)
         (void)hipGraphDestroy(static_cast<hipGraph_t>(D.graph));
     D.graph = nullptr;
+    if (D.b_graph_exec)
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(D.b_graph_exec));
+    D.b_graph_exec = nullptr;
+    if (D.b_graph)
+        (void)hipGraphDestroy(static_cast<hipGraph_t>(D.b_graph));
+    D.b_graph = nullptr;
 }
 
 std::string device_autotune(DevicePlan& D, int64_t nrhs, int reps, LaunchShape& best)

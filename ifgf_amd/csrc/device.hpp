@@ -121,6 +121,10 @@ struct DevicePlan
     // r = (up_part_cell[part] + cell) * 8 + octant spans up_dkey[2 up_doff[r] .. 2 up_doff[r+1])
     int32_t* up_dkey = nullptr;
     int32_t* up_doff = nullptr;
+    // the same tables regrouped by child cone (UpGrp records; FP32): the groups of row r span
+    // up_grp[up_goff[r] .. up_goff[r + 1])
+    int32_t* up_goff = nullptr;
+    void* up_grp = nullptr;
     int64_t* up_part_cell = nullptr;
 
     // (cell, cones of that cell, part) items of at most UP_K_CHOICES[i] cones, one list per
@@ -167,9 +171,12 @@ struct DevicePlan
 
     // work buffers of the batched mult (b_rb columns, column-minor), allocated on first use;
     // b_in / b_out stage host densities and results
-    // the single-column mult's core captured as a graph (on first use with graphs enabled)
+    // the single-column mult's core captured as a graph (on first use with graphs enabled),
+    // and the batched mult's core for the current batch buffers
     void* graph = nullptr;      // hipGraph_t
     void* graph_exec = nullptr; // hipGraphExec_t
+    void* b_graph = nullptr;
+    void* b_graph_exec = nullptr;
     void* cap_stream = nullptr; // hipStream_t of the capture
     // the second stream of the overlapped near field and its fork / join events
     void* side_stream = nullptr;
@@ -198,7 +205,7 @@ void device_free(DevicePlan& D);
 // batch buffers allocated for rb; orders 6 and 8 with tol <= 0 only (batched_supported)
 bool batched_supported(const DevicePlan& D);
 void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, int ncol,
-                  void* stream, int io);
+                  void* stream, int io, bool graph = false);
 // w: complex128 (host or device), out: complex128 (host or device); nrhs columns of length
 // N / M stored one after another.  io: IO_* bits applied while the densities are gathered and
 // the results scattered (conjugations and a sign, for adjoint products).
@@ -218,7 +225,8 @@ std::string device_mult_noalloc(DevicePlan& D, const double* w_dev, double* out_
                                 void* stream);
 // Apply a launch shape (validated; an unknown choice is an error).  Drops a captured graph.
 std::string device_set_shape(DevicePlan& D, const LaunchShape& s);
-// Drop the captured graph of the single-column mult (after a change of its kernel arguments).
+// Drop the captured graphs of the single-column and batched mults (after a change of their
+// kernel arguments).
 void device_drop_graph(DevicePlan& D);
 // Time the candidate shapes on this operator with synthetic densities (nrhs columns, each
 // candidate `reps` times, the fastest kept), one choice after another: the XCD run length on

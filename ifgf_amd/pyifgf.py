@@ -60,7 +60,8 @@ class _IfgfOperator:
         self._n = None
         self._m = None
         self._keep = None
-        # single-column mults of device tensors replay the mult's kernels as a HIP graph
+        # mults of device tensors replay the mult's kernels as a HIP graph (one captured graph
+        # for single columns, one for batches of several)
         self.use_graph = False
 
     def __del__(self, _finalizing=sys.is_finalizing):

@@ -44,10 +44,13 @@ extern "C" {
 
 /* ifgf_mult flags */
 #define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */
-#define IFGF_MULT_GRAPH 2      /* one column: replay the mult's kernels as a captured HIP graph
-                                  (captured on the first such call and reused; captured again
-                                  after ifgf_set_launch_shape / ifgf_autotune change a shape or
-                                  ifgf_set_dx changes the direction) */
+#define IFGF_MULT_GRAPH 2      /* with IFGF_DEVICE_POINTERS: replay the mult's kernels between
+                                  the gather and the scatter as a captured HIP graph, one for
+                                  the single-column mult and one for the batched mult of several
+                                  columns (each captured on its first such call and reused;
+                                  captured again after ifgf_set_launch_shape / ifgf_autotune
+                                  change a shape, ifgf_set_dx changes the direction or the batch
+                                  buffers change width) */
 
 typedef struct ifgf_operator ifgf_operator;
 
