@@ -207,3 +207,91 @@ def test_configuration4_full_size(gpu):
     print(f"configuration 4: 64 directional derivatives vs central differences {e_dir:.2e}",
           flush=True)
     assert e_dir < 5e-3
+
+
+def direct_matvec(X, sig, kappa, chunk=256):
+    """u_j = sum_{i != j} e^{i kappa |x_j - x_i|} / (4 pi |x_j - x_i|) sigma_i, FP64 with exact
+    coordinate differences (torch on the GPU, chunks of targets)."""
+    n = X.shape[0]
+    sr, si = sig.real.contiguous(), sig.imag.contiguous()
+    ur = torch.empty(n, dtype=torch.float64, device=X.device)
+    ui = torch.empty_like(ur)
+    for j0 in range(0, n, chunk):
+        diff = X[j0:j0 + chunk, None, :] - X[None, :, :]
+        d = torch.sqrt((diff * diff).sum(-1))
+        del diff
+        inv = torch.where(d > 0, 1.0 / (4 * np.pi * torch.where(d > 0, d, 1.0)), 0.0)
+        c = torch.cos(kappa * d) * inv
+        s = torch.sin(kappa * d) * inv
+        ur[j0:j0 + chunk] = c @ sr - s @ si
+        ui[j0:j0 + chunk] = c @ si + s @ sr
+    return torch.complex(ur, ui)
+
+
+def kernel_block(Y, X, kappa):
+    """e^{i kappa |y - x|} / (4 pi |y - x|) for every (y, x) pair, 0 where y = x (FP64)."""
+    diff = Y[:, None, :] - X[None, :, :]
+    d = torch.sqrt((diff * diff).sum(-1))
+    dd = torch.where(d > 0, d, 1.0)
+    e = torch.exp(1j * kappa * dd) / (4 * np.pi * dd)
+    return torch.where(d > 0, e, torch.zeros((), dtype=e.dtype, device=e.device))
+
+
+def test_configuration4_coordinate_directions(gpu):
+    """N=131,072 perturbed sphere, kappa=40: the backward's gradient with respect to the point
+    coordinates (mixed form, the gradient-kernel IFGF operators) along 64 random unit
+    directions v in R^{3N}, each moving 512 random points, against central differences
+    (h = 1e-6, FP64) of the exact loss L(x) = ||A(x) sigma - b||^2.  A(x +- h v) sigma differs
+    from the FP64 direct sum A(x) sigma only in the rows and columns of the moved points, so
+    L(x + h v) - L(x - h v) = sum_j 2 Re(conj(r_j) (D+_j - D-_j)) + |D+_j|^2 - |D-_j|^2 with
+    D+-  those exact changes (FP64 direct sums) and r = A(x) sigma - b."""
+    dev = torch.device("cuda:0")
+    N, kappa, h, M = 131072, 40.0, 1e-6, 512
+    pts = perturbed_sphere(N, 1004)
+    x0 = torch.as_tensor(pts, device=dev)
+    sig0 = torch.as_tensor(cdens(N, 2004), device=dev)
+    b = torch.as_tensor(cdens(N, 2104), device=dev)
+    x = x0.clone().requires_grad_(True)
+    print("configuration 4 (coordinates): forward and backward", flush=True)
+    u = helmholtz_apply(sig0, x, kappa=kappa, order=6, precision="mixed")
+    r = u - b
+    ((r.conj() * r).real.sum()).backward()
+    gx = x.grad.detach()
+    r0 = direct_matvec(x0, sig0, kappa) - b
+    print("configuration 4 (coordinates): direct sum", flush=True)
+
+    def change(S, XS):
+        """A(x') sigma - A(x) sigma with the points S moved to XS (all other points fixed)."""
+        xp = x0.clone()
+        xp[S] = XS
+        D = torch.zeros(N, dtype=torch.complex128, device=dev)
+        for j0 in range(0, N, 16384):  # every target, the moved sources
+            K1 = kernel_block(xp[j0:j0 + 16384], XS, kappa)
+            K0 = kernel_block(x0[j0:j0 + 16384], x0[S], kappa)
+            D[j0:j0 + 16384] = (K1 - K0) @ sig0[S]
+        # the moved targets' rows against the sources that did not move
+        rest = torch.ones(N, dtype=torch.bool, device=dev)
+        rest[S] = False
+        K1 = kernel_block(XS, x0[rest], kappa)
+        K0 = kernel_block(x0[S], x0[rest], kappa)
+        D[S] += (K1 - K0) @ sig0[rest]
+        return D
+
+    rng = np.random.default_rng(3204)
+    fds, ans = [], []
+    for i in range(64):
+        S = torch.as_tensor(np.sort(rng.choice(N, M, replace=False)), device=dev)
+        v = rng.standard_normal((M, 3))
+        v = torch.as_tensor(v / np.linalg.norm(v), device=dev)
+        dp = change(S, x0[S] + h * v)
+        dm = change(S, x0[S] - h * v)
+        diff = (2 * (r0.conj() * (dp - dm)).real + (dp.conj() * dp).real
+                - (dm.conj() * dm).real).sum()
+        fds.append(float(diff) / (2 * h))
+        ans.append(float((gx[S] * v).sum()))
+        if i % 16 == 15:
+            print(f"configuration 4 (coordinates): {i + 1} directions", flush=True)
+    e_dir = rel(np.array(ans), np.array(fds))
+    print(f"configuration 4: 64 coordinate directions vs central differences {e_dir:.2e}",
+          flush=True)
+    assert e_dir < 1e-3
