@@ -51,16 +51,18 @@ def direct_sum_torch(x, w, idx, kappa, dev):
 
 # the kernels of each phase, the first one found in a counter summary is the phase's
 KERNEL_OF_PHASE = {"near": ("k_near",), "leaf": ("k_leaf_box", "k_leaf"),
-                   "upward": ("k_up_lds", "k_up_cell"), "transform": ("k_transform_t",),
+                   "upward": ("k_up_grp", "k_up_cell"), "transform": ("k_transform_t",),
                    "far": ("k_far",)}
 # what bounds each phase's kernel below the FP32 vector roofline, from the rocprofv3 SQ
 # instruction and HBM counters of the 1M-point workloads (profiles/)
 LIMITER = {
     "near": "VALU (issue rate near the practical ceiling)",
     "leaf": "VALU (issue rate near the practical ceiling)",
-    "upward": "LDS reads of the staged child coefficients (one block per evaluation), then VALU",
+    "upward": "vector-memory latency of the dependent record / coefficient loads (about three "
+              "quarters of the wave cycles wait), then VALU",
     "transform": "LDS traffic between the axis stages and the coefficient writes to HBM",
-    "far": "VALU in the FP64 cone location, and vector-memory latency of the coefficient gathers",
+    "far": "vector-memory latency of the cell-table and coefficient gathers (more than half of "
+           "the wave cycles wait) at 4 waves per SIMD, then VALU in the FP64 cone location",
 }
 # rocprofv3 counter summaries of the 1M-point workloads, by tolerance
 COUNTERS = {

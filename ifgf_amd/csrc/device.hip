@@ -412,15 +412,15 @@ __device__ __forceinline__ void eval_cone_g(const float2* __restrict__ c,
     }
 }
 
-// eval_cone_g with the coefficients in memory, read one axis-2 plane at a time (the loads of a
-// plane are not hoisted above the previous plane's arithmetic: fewer live registers)
-template <int O0, int O1, int O2, int G>
+// eval_cone_g with the coefficients in memory, read PB axis-2 planes at a time (the loads of a
+// batch are not hoisted above the previous batch's arithmetic: fewer live registers)
+template <int O0, int O1, int O2, int G, int PB = (O2 % 2 == 0) ? 2 : 1>
 __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
                                                 const float (&t0)[G][O0], const float (&t1)[G][O1],
                                                 const float (&t2)[G][O2], float2 (&out)[G])
 {
+    static_assert(O2 % PB == 0, "whole batches of planes");
     constexpr int NP = O0 * O1;
-    constexpr int PB = (O2 % 2 == 0) ? 2 : 1; // planes per batch of loads
     float2 cp2[PB * NP];
 #pragma unroll
     for (int i2 = 0; i2 < O2; i2++)
@@ -910,8 +910,8 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
         Cx<R> val;
         if constexpr (FAST && O0 > 0)
         {
-            // FP32, compiled orders: the interpolant from its Chebyshev values, the coefficients
-            // read one row at a time
+            // FP32, compiled orders: the interpolant from its Chebyshev values, the cone's
+            // coefficients loaded first as whole float4
             float t0[1][O0], t1[1][O1], t2[1][O2];
             cheb_t<O0>(x0, t0[0]);
             cheb_t<O1>(x1, t1[0]);
