@@ -6,8 +6,9 @@
 // or (target, source box) and applied to the RB columns: the P2P and factored kernel values,
 // the upward table entries and cone lookups, the far-field cone location and centre factor.
 // The Clenshaw sums read a coefficient row of RB columns per load (coalesced across the RHS
-// lanes).  Compile-time interpolation orders only (orders 6 and 8 with tol <= 0); other
-// orders run the single-column mult per column.
+// lanes).  Compile-time interpolation orders only (orders 6 and 8 with tol <= 0, and the
+// orders (6,8,7)/(3,5,4) of the tolerance estimate at order 6); other orders run the
+// single-column mult per column.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,18 +69,67 @@ template <typename R> __global__ void k_zero_b(Cx<R>* p, int64_t n)
         p[i] = {R(0), R(0)};
 }
 
-// value of column r of the fine interpolant in slot `slot` (coefficients RB apart)
+// value of column r of the fine interpolant in slot `slot` (coefficients RB apart).  Up to 50
+// coefficients: loaded, then the Clenshaw sums; more (orders (3,5,4)): the sum of
+// c_ijk T_i(x0) T_j(x1) T_k(x2) from the Chebyshev values, a row of coefficients at a time (the
+// coefficients of a whole cone would not fit the registers)
 template <int O0, int O1, int O2, typename R, int RB>
 __device__ __forceinline__ Cx<R> fine_eval_s(const Cx<R>* __restrict__ F, int slot, int r, R x0,
                                              R x1, R x2)
 {
     constexpr int N = O0 * O1 * O2;
-    Cx<R> c[N];
     const Cx<R>* p = F + int64_t(slot) * N * RB + r;
+    if constexpr (N <= 50)
+    {
+        Cx<R> c[N];
 #pragma unroll
-    for (int i = 0; i < N; i++)
-        c[i] = p[i * RB];
-    return clen3c<O0, O1, O2, R>(c, x0, x1, x2);
+        for (int i = 0; i < N; i++)
+            c[i] = p[i * RB];
+        return clen3c<O0, O1, O2, R>(c, x0, x1, x2);
+    }
+    else
+    {
+        R t0[O0], t1[O1], t2[O2];
+        t0[0] = t1[0] = t2[0] = R(1);
+        if constexpr (O0 > 1)
+            t0[1] = x0;
+        if constexpr (O1 > 1)
+            t1[1] = x1;
+        if constexpr (O2 > 1)
+            t2[1] = x2;
+#pragma unroll
+        for (int k = 2; k < O0; k++)
+            t0[k] = R(2) * x0 * t0[k - 1] - t0[k - 2];
+#pragma unroll
+        for (int k = 2; k < O1; k++)
+            t1[k] = R(2) * x1 * t1[k - 1] - t1[k - 2];
+#pragma unroll
+        for (int k = 2; k < O2; k++)
+            t2[k] = R(2) * x2 * t2[k - 1] - t2[k - 2];
+        Cx<R> out = {0, 0};
+#pragma unroll
+        for (int i2 = 0; i2 < O2; i2++)
+        {
+            Cx<R> pl = {0, 0};
+#pragma unroll
+            for (int i1 = 0; i1 < O1; i1++)
+            {
+                Cx<R> sm = {0, 0};
+#pragma unroll
+                for (int i0 = 0; i0 < O0; i0++)
+                {
+                    const Cx<R> c = p[(i0 + O0 * (i1 + O1 * i2)) * RB];
+                    sm.x += c.x * t0[i0];
+                    sm.y += c.y * t0[i0];
+                }
+                pl.x += sm.x * t1[i1];
+                pl.y += sm.y * t1[i1];
+            }
+            out.x += pl.x * t2[i2];
+            out.y += pl.y * t2[i2];
+        }
+        return out;
+    }
 }
 
 // near field (ifgfoperator.hpp:900-960): as k_near (source leaves staged in LDS, lanes =
@@ -853,7 +903,8 @@ void run_batched_core(DevicePlan& D, hipStream_t st)
 {
     constexpr bool PH = KIND != KL;
     constexpr bool P6 = O1 == 3;
-    constexpr int NI = sizeof(R) == 4 ? (P6 ? 4 : 2) : (P6 ? 2 : 1);
+    constexpr bool C3 = O0 == 3; // orders (6,8,7)/(3,5,4)
+    constexpr int NI = C3 ? 1 : (sizeof(R) == 4 ? (P6 ? 4 : 2) : (P6 ? 2 : 1));
     const KParams kp = kparams(D);
     const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
     Cx<R>* ws = static_cast<Cx<R>*>(D.b_ws);
@@ -939,6 +990,8 @@ void run_batched_core(DevicePlan& D, hipStream_t st)
         {
             if constexpr (P6)
                 launch_transform_b<R, 3, 6, 6, 2, 3, 3, NI, RB>(D, st, c0, c1, V, F);
+            else if constexpr (C3)
+                launch_transform_b<R, 6, 8, 7, 3, 5, 4, NI, RB>(D, st, c0, c1, V, F);
             else
                 launch_transform_b<R, 5, 8, 8, 2, 5, 5, NI, RB>(D, st, c0, c1, V, F);
             pipe.fork();
@@ -1018,17 +1071,22 @@ template <typename R, int RB>
 void run_order(DevicePlan& D, const double* w, double* o, int ncol, hipStream_t st, int io,
                bool graph)
 {
-    if (order_class(D) == 1)
-        run_kind<R, RB, 2, 3, 3>(D, w, o, ncol, st, io, graph);
-    else
-        run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io, graph);
+    switch (order_class(D))
+    {
+    case 1:
+        return run_kind<R, RB, 2, 3, 3>(D, w, o, ncol, st, io, graph);
+    case 2:
+        return run_kind<R, RB, 2, 5, 5>(D, w, o, ncol, st, io, graph);
+    default:
+        return run_kind<R, RB, 3, 5, 4>(D, w, o, ncol, st, io, graph);
+    }
 }
 } // namespace
 
 bool batched_supported(const DevicePlan& D)
 {
     const int c = order_class(D);
-    return c == 1 || c == 2;
+    return c == 1 || c == 2 || c == 3;
 }
 
 void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, int ncol,

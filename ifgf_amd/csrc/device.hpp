@@ -202,7 +202,7 @@ struct DevicePlan
 std::string device_upload(const Plan& P, int precision, DevicePlan& D);
 void device_free(DevicePlan& D);
 // batched mult (batched.hip): ncol <= rb columns (rb = 8 or 32) of device densities with the
-// batch buffers allocated for rb; orders 6 and 8 with tol <= 0 only (batched_supported)
+// batch buffers allocated for rb; the compiled order classes only (batched_supported)
 bool batched_supported(const DevicePlan& D);
 void mult_batched(DevicePlan& D, const double* w_dev, double* out_dev, int rb, int ncol,
                   void* stream, int io, bool graph = false);

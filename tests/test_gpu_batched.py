@@ -266,3 +266,24 @@ def test_batched_and_merged_graph_replays_match_eager(gpu, prec):
             torch.cuda.synchronize()
             assert torch.equal(u.cpu(), eager), (case, rep)
         o.use_graph = False
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batched_tolerance_order_class(gpu, prec):
+    """The batched kernels at the orders (6,8,7)/(3,5,4) that the tolerance estimate picks for
+    order 6 at tol 1e-5 (N = 12,000, kappa = 16): 10 columns in one pass against the
+    single-column mult column by column and two columns against the oracle."""
+    x = sphere(12000, 100 + 12000 % 97)
+    W = np.stack([cdens(12000, 170 + i) for i in range(10)], axis=1)
+    print(f"batched tolerance class prec={prec}", flush=True)
+    op = ifgf_amd.HelmholtzIfgfOperator(16.0, 64, 6, 1, 1e-5, precision=prec)
+    op.init(x.T, x.T)
+    o = op.plan_orders()
+    assert o["high"].tolist() == [6, 8, 7] and o["low"].tolist() == [3, 5, 4], o
+    U = op.mult(W)
+    ref = single_columns(op, W)
+    for i in range(10):
+        assert rel(U[:, i], ref[:, i]) < TOL_SAME[prec], (i, rel(U[:, i], ref[:, i]))
+    orc = po.Oracle(po.HELMHOLTZ, 16.0, 64, 6, 1, 1e-5).init(x, x)
+    for i in (0, 9):
+        assert rel(U[:, i], orc.mult(W[:, i])) < TOL_ORACLE[prec]
