@@ -20,6 +20,7 @@ HELMHOLTZ, MODHELMHOLTZ, LAPLACE, DOUBLELAYER, COMBINED, GRAD = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_PLAN_ONLY = 1
 INIT_HOST_PLAN = 2
+INIT_LEVEL_RING = 4
 DEVICE_POINTERS = 1
 MULT_GRAPH = 2
 

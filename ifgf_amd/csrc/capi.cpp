@@ -132,7 +132,8 @@ extern "C"
             op->planned = true;
             if (plan_only)
                 return IFGF_OK;
-            e = device_upload(op->plan, op->precision, op->dev);
+            e = device_upload(op->plan, op->precision, op->dev,
+                              (flags & IFGF_INIT_LEVEL_RING) ? 1 : 0);
             if (!e.empty())
                 return fail(op, IFGF_EINTERNAL, e);
             op->on_device = true;
@@ -366,6 +367,7 @@ extern "C"
             c[12]++;
             c[13] += int64_t(pp.U.entries.size());
         }
+        c[14] = op->dev.ring ? 1 : 0;
         return IFGF_OK;
     }
 

@@ -778,9 +778,12 @@ template <class NearFn, class FarFn> struct SidePipe
     FarFn far;   // far(target level, stream)
     bool forked = false;
     int next;    // the next far group to enqueue (deepest source level first)
+    // level storage (DevicePlan::ring) on one stream: the far groups of a source level run as
+    // soon as its coefficients are written, before the level two above overwrites them
+    bool inline_far;
     SidePipe(DevicePlan& d, hipStream_t s, int mode, NearFn n, FarFn f)
         : D(d), st(s), on(mode != 0 && d.side_stream), near_side(on && mode == 1), near(n),
-          far(f), next(0)
+          far(f), next(0), inline_far(!on && d.ring)
     {
     }
     hipStream_t side() const { return static_cast<hipStream_t>(D.side_stream); }
@@ -800,10 +803,16 @@ template <class NearFn, class FarFn> struct SidePipe
     }
     void ready(int done)
     {
+        const int ng = int(D.far_groups.size());
+        if (inline_far)
+        {
+            while (next < ng && D.far_groups[next].src_level >= done)
+                far(next++, st);
+            return;
+        }
         if (!on || !forked)
             return;
         bool waited = false;
-        const int ng = int(D.far_groups.size());
         while (next < ng && D.far_groups[next].src_level >= done)
         {
             if (!waited)
@@ -815,6 +824,16 @@ template <class NearFn, class FarFn> struct SidePipe
             }
             far(next++, side());
         }
+        if (D.ring)
+            HIP_OK(hipEventRecord(static_cast<hipEvent_t>(D.ev_far_done[done]), side()));
+    }
+    // level storage on two streams: the main stream waits until the far groups of source level
+    // `lvl` (and deeper) have read their coefficients, before their buffer is written again
+    void reuse(int lvl)
+    {
+        if (!on || !D.ring || !forked || lvl >= D.levels)
+            return;
+        HIP_OK(hipStreamWaitEvent(st, static_cast<hipEvent_t>(D.ev_far_done[lvl]), 0));
     }
     void finish()
     {

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1287,6 +1288,23 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
     Cx<R>* res = static_cast<Cx<R>*>(sg.res);
     Cx<R>* V = static_cast<Cx<R>*>(D.V);
     Cx<R>* F = static_cast<Cx<R>*>(D.F);
+    // the values of level l and the fine coefficients of level l: the whole arrays, or with level
+    // storage the level's buffer addressed as if it held every level (the kernels index by
+    // global cone id and fine slot, and touch only the level's range)
+    auto Vl = [&](int l) -> Cx<R>*
+    {
+        if (!D.ring)
+            return V;
+        return reinterpret_cast<Cx<R>*>(reinterpret_cast<uintptr_t>(D.V) -
+                                        uintptr_t(D.cone_lvl_off[l]) * D.n_hi * sizeof(Cx<R>));
+    };
+    auto Fl = [&](int l) -> Cx<R>*
+    {
+        if (!D.ring)
+            return F;
+        return reinterpret_cast<Cx<R>*>(reinterpret_cast<uintptr_t>(D.F_bank[l & 1]) -
+                                        uintptr_t(D.fine_lvl_off[l]) * D.n_lo * sizeof(Cx<R>));
+    };
     // near and far fields: SidePipe (overlapped with the upward chain on the side stream when
     // the shape asks for it; phase timing runs everything in order on one stream)
     const int overlap = (pt.on || !D.n_near_items || !sg.side) ? 0 : D.shape.overlap;
@@ -1304,35 +1322,56 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
         [&](int g, hipStream_t s)
         {
             const DevicePlan::FarGroup& G = D.far_groups[g];
-            launch_up_far<PH, R>(D, s, kp, tabs, -1, g, G.item0, G.item1, F, V, res);
+            launch_up_far<PH, R>(D, s, kp, tabs, -1, g, G.item0, G.item1, Fl(G.src_level), V,
+                                 res);
         });
     if (overlap != 1)
         pipe.fork(); // the near field first, on the main stream (inside the timed near phase)
     pt.end();
-    pt.begin(1);
-    if (D.n_leaf_boxes)
+    if (!D.ring && D.n_leaf_boxes)
+    {
+        // every level's leaf cones at once (with level storage: a level at a time, below)
+        pt.begin(1);
         k_leaf_box<KIND, R><<<dim3(unsigned(D.n_leaf_boxes)), dim3(256), 0, st>>>(
             D.leaf_boxes, D.n_leaf_boxes, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
             D.normals, ws, tabs, V, kp);
-    pt.end();
+        pt.end();
+    }
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
+        if (D.ring && D.leaf_lvl_off[l + 1] > D.leaf_lvl_off[l])
+        {
+            pt.begin(1);
+            const int64_t a = D.leaf_lvl_off[l], nb = D.leaf_lvl_off[l + 1] - a;
+            k_leaf_box<KIND, R><<<dim3(unsigned(nb)), dim3(256), 0, st>>>(
+                D.leaf_boxes + a, nb, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
+                D.normals, ws, tabs, Vl(l), kp);
+            pt.end();
+        }
         if (u1 > u0 || D.up_item_off[l + 1] > D.up_item_off[l])
         {
             pt.begin(2);
-            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, F, V, res);
+            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, Fl(l + 1), Vl(l), res);
             pt.end();
         }
         if (c1 > c0)
         {
+            pipe.reuse(l + 2); // level storage: F_bank[l & 1] held level l + 2
             pt.begin(3);
-            launch_transform<R>(D, st, kp, c0, c1, V, F);
+            launch_transform<R>(D, st, kp, c0, c1, Vl(l), Fl(l));
             pt.end();
             pipe.fork();
         }
-        pipe.ready(l);
+        if (D.ring && pt.on)
+        {
+            pt.begin(4); // level storage: this level's far groups run here
+            pipe.ready(l);
+            pt.end();
+        }
+        else
+            pipe.ready(l);
     }
     pt.begin(4);
     pipe.finish();
@@ -1461,7 +1500,8 @@ void device_free(DevicePlan& D)
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
-                    D.F, D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in, D.b_out, D.h_in, D.h_out};
+                    D.F, D.F_bank[0], D.F_bank[1], D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in,
+                    D.b_out, D.h_in, D.h_out};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1484,10 +1524,13 @@ void device_free(DevicePlan& D)
     for (void* e : D.ev_level)
         if (e)
             (void)hipEventDestroy(static_cast<hipEvent_t>(e));
+    for (void* e : D.ev_far_done)
+        if (e)
+            (void)hipEventDestroy(static_cast<hipEvent_t>(e));
     D = DevicePlan();
 }
 
-std::string device_upload(const Plan& P, int precision, DevicePlan& D)
+std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
 {
     try
     {
@@ -1897,6 +1940,19 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
                                   int32_t(P.cone_off[g + 1] - P.cone_off[g]), 0});
             D.leaf_boxes = dupload(lb, tot);
             D.n_leaf_boxes = int64_t(lb.size());
+            // per level (boxes are level-major): the range of the leaf boxes, of the fine slots
+            D.leaf_lvl_off.assign(D.levels + 1, 0);
+            for (const WorkItem& w : lb)
+                for (int l = 0; l < D.levels; l++)
+                    if (w.box >= P.S.level_off[l] && w.box < P.S.level_off[l + 1])
+                        D.leaf_lvl_off[l + 1]++;
+            for (int l = 0; l < D.levels; l++)
+                D.leaf_lvl_off[l + 1] += D.leaf_lvl_off[l];
+            D.fine_lvl_off.assign(D.levels + 1, 0);
+            for (int l = 0; l <= D.levels; l++)
+                D.fine_lvl_off[l] = D.cone_lvl_off[l] < D.n_cones
+                                        ? int64_t(P.cone_fine[D.cone_lvl_off[l]])
+                                        : D.n_fine;
         }
         D.up_cones = dupload(up, tot);
         // far / near lists grouped by target box (pairs are sorted by target)
@@ -1981,8 +2037,36 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D)
         const size_t cs = precision == PREC_FP64 ? 16 : 8;
         D.wsorted = dalloc<unsigned char>(cs * D.nsrc, tot);
         D.result = dalloc<unsigned char>(cs * D.ntgt, tot);
-        D.V = dalloc<unsigned char>(cs * size_t(std::max<int64_t>(1, D.n_cones)) * D.n_hi, tot);
-        D.F = dalloc<unsigned char>(cs * size_t(std::max<int64_t>(1, D.n_fine)) * D.n_lo, tot);
+        const size_t full = cs * (size_t(std::max<int64_t>(1, D.n_cones)) * D.n_hi +
+                                  size_t(std::max<int64_t>(1, D.n_fine)) * D.n_lo);
+        size_t v_lvl = 1, f_lvl[2] = {1, 1};
+        for (int l = 0; l < D.levels; l++)
+        {
+            v_lvl = std::max(v_lvl, size_t(D.cone_lvl_off[l + 1] - D.cone_lvl_off[l]) * D.n_hi);
+            f_lvl[l & 1] =
+                std::max(f_lvl[l & 1], size_t(D.fine_lvl_off[l + 1] - D.fine_lvl_off[l]) * D.n_lo);
+        }
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const size_t margin = size_t(4) << 30; // batch buffers, torch, the caller's vectors
+        D.ring = ring == 1 || (full + margin > free_b && D.levels > 2);
+        if (D.ring)
+        {
+            if (cs * (v_lvl + f_lvl[0] + f_lvl[1]) + margin > free_b)
+                throw std::runtime_error(
+                    "the values of one level and the fine coefficients of two levels (" +
+                    std::to_string(cs * (v_lvl + f_lvl[0] + f_lvl[1]) >> 20) +
+                    " MiB) do not fit the free device memory (" + std::to_string(free_b >> 20) +
+                    " MiB)");
+            D.V = dalloc<unsigned char>(cs * v_lvl, tot);
+            D.F_bank[0] = dalloc<unsigned char>(cs * f_lvl[0], tot);
+            D.F_bank[1] = dalloc<unsigned char>(cs * f_lvl[1], tot);
+        }
+        else
+        {
+            D.V = dalloc<unsigned char>(cs * size_t(std::max<int64_t>(1, D.n_cones)) * D.n_hi, tot);
+            D.F = dalloc<unsigned char>(cs * size_t(std::max<int64_t>(1, D.n_fine)) * D.n_lo, tot);
+        }
         HIP_OK(hipDeviceSynchronize());
     }
     catch (const std::exception& e)
@@ -2062,10 +2146,17 @@ std::string device_mult(DevicePlan& D, const void* w, void* out, int64_t nrhs, b
                 HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
                 e = ev;
             }
+            D.ev_far_done.assign(std::max(1, D.levels), nullptr);
+            for (void*& e : D.ev_far_done)
+            {
+                hipEvent_t ev;
+                HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                e = ev;
+            }
         }
         // several columns: the batched pipeline (8 or 32 columns per pass) when the orders
         // have compiled kernels and the batch buffers fit
-        if (nrhs >= 2 && batched_supported(D) && std::getenv("IFGF_NO_BATCH") == nullptr)
+        if (nrhs >= 2 && batched_supported(D) && !D.ring && std::getenv("IFGF_NO_BATCH") == nullptr)
         {
             int rb = D.shape.rb > 0 ? D.shape.rb : (nrhs <= 8 ? 8 : 32);
             bool ok = ensure_batch(D, rb, !device_ptrs);

@@ -167,6 +167,15 @@ struct DevicePlan
     void* result = nullptr;   // M complex (R)
     void* V = nullptr;        // n_cones * n_hi complex (R)
     void* F = nullptr;        // n_fine * n_lo complex (R)
+    // level storage (ring): instead of every level's values and coefficients, V holds the values
+    // of one level and F_bank[l & 1] the fine coefficients of level l (a level's coefficients
+    // are read only by the upward pass of the level above and the far field of that source
+    // level); for operators whose coefficients of all levels exceed the device memory
+    bool ring = false;
+    void* F_bank[2] = {nullptr, nullptr};
+    std::vector<int64_t> fine_lvl_off;    // per level, global fine-slot range
+    std::vector<int64_t> leaf_lvl_off;    // per level, range of leaf_boxes
+    std::vector<void*> ev_far_done;       // ring on two streams: far groups of a source level done
     int64_t bytes_total = 0;
 
     // work buffers of the batched mult (b_rb columns, column-minor), allocated on first use;
@@ -199,7 +208,9 @@ struct DevicePlan
     int64_t b_bytes = 0;
 };
 
-std::string device_upload(const Plan& P, int precision, DevicePlan& D);
+// ring: 0 = every level resident when it fits the device memory, else level storage;
+// 1 = level storage (DevicePlan::ring)
+std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring = 0);
 void device_free(DevicePlan& D);
 // batched mult (batched.hip): ncol <= rb columns (rb = 8 or 32) of device densities with the
 // batch buffers allocated for rb; the compiled order classes only (batched_supported)

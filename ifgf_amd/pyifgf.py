@@ -75,7 +75,7 @@ class _IfgfOperator:
 
     # IfgfOperator::init (ifgfoperator.hpp:65-106)
     def init(self, srcs, targets, cutOff=None, normals=None, *, plan_only=False,
-             host_plan=False):
+             host_plan=False, level_ring=False):
         s = _points(srcs, "srcs")
         t = _points(targets, "targets")
         n = None if normals is None else _points(normals, "normals")
@@ -85,7 +85,8 @@ class _IfgfOperator:
         rc = lib().ifgf_init(self._h, s.ctypes.data, s.shape[0], t.ctypes.data, t.shape[0],
                              None if n is None else n.ctypes.data, cb, None,
                              (_lib.INIT_PLAN_ONLY if plan_only else 0)
-                             | (_lib.INIT_HOST_PLAN if host_plan else 0))
+                             | (_lib.INIT_HOST_PLAN if host_plan else 0)
+                             | (_lib.INIT_LEVEL_RING if level_ring else 0))
         check(rc, self._h, "init")
         self._n, self._m = s.shape[0], t.shape[0]
 
@@ -147,7 +148,7 @@ class _IfgfOperator:
         check(lib().ifgf_plan_counts(self._h, c.ctypes.data), self._h, "plan_counts")
         keys = ["src_levels", "src_boxes", "tgt_levels", "tgt_boxes", "far_pairs", "near_pairs",
                 "coarse_cones", "fine_cones", "fine_uncovered", "dropped", "device_bytes",
-                "fine_ids", "up_table_levels", "up_table_entries"]
+                "fine_ids", "up_table_levels", "up_table_entries", "level_ring"]
         return {k: int(v) for k, v in zip(keys, c)}
 
     def plan_orders(self):

@@ -41,6 +41,12 @@ extern "C" {
 #define IFGF_INIT_HOST_PLAN 2 /* sort the points, build the octrees and mark the active cones
                                  with the host code (by default a device init does these on
                                  the GPU; both give the same plan) */
+#define IFGF_INIT_LEVEL_RING 4 /* keep the interpolation values of one level and the fine
+                                  coefficients of two levels on the device instead of every
+                                  level's (same results; the far field of a level runs as soon
+                                  as its coefficients are written, and mults of several columns
+                                  run one column at a time).  Chosen by init when every level's
+                                  do not fit the free device memory. */
 
 /* ifgf_mult flags */
 #define IFGF_DEVICE_POINTERS 1 /* w and out are device pointers; work is enqueued on stream */
@@ -126,7 +132,8 @@ int ifgf_device_count(void);
  *         [4] far pairs [5] near pairs [6] coarse cones [7] fine cones (covered)
  *         [8] fine cones active but not covered [9] dropped (cutOff) pairs
  *         [10] device bytes [11] fine cone ids listed
- *         [12] levels with precomputed upward geometry [13] entries of those tables */
+ *         [12] levels with precomputed upward geometry [13] entries of those tables
+ *         [14] level storage (IFGF_INIT_LEVEL_RING) in use; the array holds 15 values */
 int ifgf_plan_counts(const ifgf_operator* op, int64_t* counts);
 /* orders: base[3], coarse (high) order[3], fine (low) order[3], factor[3] */
 int ifgf_plan_orders(const ifgf_operator* op, int* out12);

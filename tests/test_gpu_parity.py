@@ -244,3 +244,66 @@ def test_normals_kernels_match_direct_sum(gpu, prec):
         err = rel(u[idx], d)
         print(f"normals kind={kind} {prec}: rel. error vs direct sum {err:.3e}", flush=True)
         assert err < 3e-3, (kind, err)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_level_storage_matches_resident_levels(gpu, prec):
+    """Level storage (IFGF_INIT_LEVEL_RING: the values of one level and the fine coefficients of
+    two levels on the device, far groups run as their level's coefficients are written) gives
+    the results of the resident plan bit for bit: every overlap mode, graph replays, the
+    allocation-free mult, several columns (one at a time), targets other than the sources and
+    the compiled and runtime-order classes."""
+    import os
+
+    import torch
+    from ifgf_amd import _lib
+    dev = torch.device("cuda:0")
+    cases = [
+        (ifgf_amd.HelmholtzIfgfOperator, (20.0, 64, 6, 1, -1), sphere(20000, 180), None),
+        (ifgf_amd.HelmholtzIfgfOperator, (16.0, 64, 6, 1, 1e-5), sphere(12000, 100 + 12000 % 97),
+         None),
+        (ifgf_amd.GradHelmholtzIfgfOperator, (1.0 + 14.0j, 64, 7, 1, -1), sphere(9000, 181),
+         sphere(4000, 182) * 1.4),
+    ]
+    for cls, args, x, y in cases:
+        tgt = x if y is None else y
+        ops = []
+        for ring in (False, True):
+            op = cls(*args, precision=prec)
+            if cls is ifgf_amd.GradHelmholtzIfgfOperator:
+                op.setDx(1)
+            op.init(x.T, tgt.T, level_ring=ring)
+            ops.append(op)
+        a, b = ops
+        assert a.plan_counts()["level_ring"] == 0 and b.plan_counts()["level_ring"] == 1
+        print(f"level storage {cls.__name__} {args} prec={prec} orders {b.plan_orders()['high']}",
+              flush=True)
+        w = torch.as_tensor(cdens(len(x), 183), device=dev)
+        for ov in (0, 1, 2):
+            for op in (a, b):
+                sh = op.launch_shape
+                sh["overlap"] = ov
+                op.launch_shape = sh
+            ua = a.mult(w)
+            ub = b.mult(w)
+            torch.cuda.synchronize()
+            assert torch.equal(ua, ub), (cls.__name__, ov)
+        b.use_graph = True
+        for _ in range(2):
+            ug = b.mult(w)
+            torch.cuda.synchronize()
+            assert torch.equal(ug, ua)
+        b.use_graph = False
+        out = torch.empty(len(tgt), dtype=torch.complex128, device=dev)
+        rc = _lib.lib().ifgf_mult_noalloc(b._h, w.data_ptr(), out.data_ptr(),
+                                          torch.cuda.current_stream().cuda_stream)
+        assert rc == _lib.IFGF_OK
+        torch.cuda.synchronize()
+        assert torch.equal(out, ua)
+        W = np.stack([cdens(len(x), 184 + i) for i in range(3)], axis=1)
+        os.environ["IFGF_NO_BATCH"] = "1"
+        try:
+            ref = a.mult(W)
+        finally:
+            del os.environ["IFGF_NO_BATCH"]
+        assert np.array_equal(b.mult(W), ref)
