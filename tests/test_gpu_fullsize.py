@@ -98,32 +98,42 @@ def test_combined_field_torus_mixed_vs_fp64(gpu):
 
 @pytest.mark.timeout(1200)
 def test_configuration5_torus_4M(gpu):
-    """N = 4,194,304 on the torus, combined field, kappa = 200, order 8, tolerance 1e-5, mixed
-    form (the orders the tolerance path picks, (8,8,8)/(5,5,5), give 12 M coarse and 169 M fine
-    cones: 218 GB of values and coefficients in the mixed form; the FP64 form would need twice
-    that, beyond one GPU's HBM): error against an FP64 direct sum on 200 sampled targets."""
+    """N = 4,194,304 on the torus, combined field, kappa = 200, order 8, tolerance 1e-5 (the
+    orders the tolerance path picks, (8,8,8)/(5,5,5), give 12 M coarse and 169 M fine cones):
+    the mixed form with every level resident (218 GB of values and coefficients), then the FP64
+    form, whose 436 GB of values and coefficients do not fit one GPU, with level storage (the
+    values of one level and the fine coefficients of two levels, about 260 GB), chosen by init;
+    both against an FP64 direct sum on 200 sampled targets, and against each other."""
+    import time
+
     import torch
     from conftest import torus
     from oracle import pyoracle as po
     N, kappa = 4194304, 200.0
     x, n = torus(N, 1005)
     w = cdens(N, 2005)
-    print("configuration 5: plan", flush=True)
-    op = ifgf_amd.CombinedFieldHelmholtzIfgfOperator(kappa, 64, 8, 1, 1e-5)
-    op.init(x.T, x.T, n.T)
-    c = op.plan_counts()
-    print(f"configuration 5: {op.plan_orders()} {c}", flush=True)
     wd = torch.as_tensor(w, device="cuda:0")
-    op.mult(wd)
-    torch.cuda.synchronize()
-    import time
-    t0 = time.time()
-    u = op.mult(wd)
-    torch.cuda.synchronize()
-    print(f"configuration 5: mult {1e3 * (time.time() - t0):.0f} ms", flush=True)
-    u = u.cpu().numpy()
+    u = {}
+    for prec in ("mixed", "fp64"):
+        print(f"configuration 5 {prec}: plan", flush=True)
+        op = ifgf_amd.CombinedFieldHelmholtzIfgfOperator(kappa, 64, 8, 1, 1e-5, precision=prec)
+        op.init(x.T, x.T, n.T)
+        c = op.plan_counts()
+        print(f"configuration 5 {prec}: {op.plan_orders()} {c}", flush=True)
+        assert c["level_ring"] == (1 if prec == "fp64" else 0)
+        op.mult(wd)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        u[prec] = op.mult(wd)
+        torch.cuda.synchronize()
+        print(f"configuration 5 {prec}: mult {1e3 * (time.time() - t0):.0f} ms", flush=True)
+        u[prec] = u[prec].cpu().numpy()
+        del op
+        torch.cuda.empty_cache()
     idx = np.random.default_rng(3005).choice(N, 200, replace=False)
     ref = po.Oracle(po.COMBINED, kappa, 64, 8, 1, 1e-5).direct_sum(x, x, w, idx, normals=n)
-    err = np.linalg.norm(u[idx] - ref) / np.linalg.norm(ref)
-    print(f"configuration 5: rel. error vs direct sum {err:.2e}", flush=True)
-    assert err < 1e-4
+    err = {p: np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref) for p, v in u.items()}
+    d = np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"])
+    print(f"configuration 5: rel. error vs direct sum mixed {err['mixed']:.2e} fp64 "
+          f"{err['fp64']:.2e}; mixed vs fp64 {d:.2e}", flush=True)
+    assert err["mixed"] < 1e-4 and err["fp64"] < 1e-4 and d < 1e-5
