@@ -287,3 +287,70 @@ def test_batched_tolerance_order_class(gpu, prec):
     orc = po.Oracle(po.HELMHOLTZ, 16.0, 64, 6, 1, 1e-5).init(x, x)
     for i in (0, 9):
         assert rel(U[:, i], orc.mult(W[:, i])) < TOL_ORACLE[prec]
+
+
+def test_configuration3_one_operator(gpu):
+    """Configuration 3 as one operator: the oblate spheroid (N = 262,144, kappa = 64) and 16
+    spheres (N log-uniform in [4096, 262144], kappa uniform in [8, 64]) merged, 32 density
+    columns per part, one batched pass (one launch per level for every geometry and column),
+    FP64 form: every part against its own operator (same columns) to 1e-12, and the three
+    smallest spheres against the oracle on two columns to 1e-11; the mixed form against the
+    FP64 one; timings of the merged pass against the 17 separate batched mults."""
+    import time
+
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1013)
+    Ns = np.floor(4096 * 64.0 ** rng.random(16)).astype(int)
+    ks = rng.uniform(8, 64, 16)
+    geos = [(spheroid(262144, 1003), 64.0)] + [(sphere(int(n), 1100 + i), float(k))
+                                               for i, (n, k) in enumerate(zip(Ns, ks))]
+    print(f"configuration 3 one operator: N = {[len(g[0]) for g in geos]}", flush=True)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ops, Ws = [], []
+        for i, (x, k) in enumerate(geos):
+            op = ifgf_amd.HelmholtzIfgfOperator(k, 64, 6, 1, -1, precision=prec)
+            op.init(x.T, x.T)
+            ops.append(op)
+            r = np.random.default_rng(2200 + i)
+            Ws.append(r.standard_normal((len(x), 32)) + 1j * r.standard_normal((len(x), 32)))
+        m = ifgf_amd.merge(ops)
+        Wd = torch.as_tensor(np.concatenate(Ws), device=dev)
+        m.mult(Wd)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        U = m.mult(Wd)
+        torch.cuda.synchronize()
+        tm = time.time() - t0
+        wds = [torch.as_tensor(W, device=dev) for W in Ws]
+        for op, W in zip(ops, wds):
+            op.mult(W)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        us = [op.mult(W) for op, W in zip(ops, wds)]
+        torch.cuda.synchronize()
+        ts = time.time() - t0
+        print(f"configuration 3 one operator {prec}: merged 32 columns {tm * 1e3:.1f} ms, 17 "
+              f"separate batched mults {ts * 1e3:.1f} ms", flush=True)
+        U = U.cpu().numpy()
+        res[prec] = U
+        at = 0
+        for (x, k), ui in zip(geos, us):
+            part = U[at:at + len(x)]
+            at += len(x)
+            tol = 1e-12 if prec == "fp64" else TOL_SAME["mixed"]
+            assert rel(part, ui.cpu().numpy()) < tol
+        if prec == "fp64":
+            order = np.argsort([len(g[0]) for g in geos[1:]])[:3] + 1
+            offs = np.cumsum([0] + [len(g[0]) for g in geos])
+            for j in order:
+                x, k = geos[j]
+                orc = po.Oracle(po.HELMHOLTZ, k, 64, 6, 1, -1).init(x, x)
+                for c in (0, 31):
+                    assert rel(U[offs[j]:offs[j + 1], c], orc.mult(Ws[j][:, c])) < 1e-11
+        del m, ops
+        torch.cuda.empty_cache()
+    d = rel(res["mixed"], res["fp64"])
+    print(f"configuration 3 one operator: mixed vs fp64 {d:.2e}", flush=True)
+    assert d < 2e-5
