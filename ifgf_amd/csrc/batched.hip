@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdint>
 #include <stdexcept>
 #include <vector>
@@ -884,11 +885,144 @@ void launch_far_b(DevicePlan& D, hipStream_t st, const KParams& kp, int l, int64
         D.fine_ang, 0, 0, res, kp);
 }
 
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+// one-wave workgroups: the wave's LDS writes complete before its later LDS reads (of other
+// lanes' values) issue; the memory clobber keeps the compiler from moving LDS accesses across
+__device__ __forceinline__ void lds_fence()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+
+// values -> fine coefficients at orders (3,6,6)/(2,3,3) on FP32 MFMA (v_mfma_f32_16x16x4f32),
+// one wave per (cone, 16 real columns): the three axis maps as 16 x 16 x 4 tiles, axis 1, then
+// axis 2, then axis 0 (171 MFMAs when every sub-cone is active; the 6 x 3 and 9 x 6 maps fill
+// 28% and 42% of a tile). Tile operands: lane l holds A[l % 16][l / 16] (map row, input index)
+// and B[l / 16][l % 16] (input index, column); the result tile row 4 (l / 16) + i, column
+// l % 16 is element i of lane l. The same map as k_transform_b up to rounding (the axes in
+// another order), and slower than it on the 32-column spheroid (8.5 against 6.7 ms per mult):
+// the pass moves 19 GB (values read, coefficients of the active sub-cones written) and the
+// arithmetic is not what bounds it. Kept for that comparison: it runs only when the
+// environment variable IFGF_TRANSFORM_MFMA is set when a batched pass is enqueued or captured.
+template <int RB>
+__global__ __launch_bounds__(64) void k_transform_mf(int64_t c0, int64_t c1,
+                                                     const ConeDev* __restrict__ cones,
+                                                     const float* __restrict__ V,
+                                                     float* __restrict__ Fout,
+                                                     const float* __restrict__ W, int xlg)
+{
+    constexpr int NC = 2 * RB, NT = NC / 16;
+    constexpr int S1 = 100, S2 = 68; // padded row strides: lane groups 16 banks apart
+    __shared__ float Y1[27 * S1];    // (t0, j1) x (t2, column)
+    __shared__ float Y2[9 * S2];     // per j1: j2 x (t0, column)
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    const int64_t blk = xcd_block(xlg);
+    const int64_t cone = c0 + blk / NT;
+    if (cone >= c1)
+        return;
+    const int n = int(blk % NT) * 16 + c;
+    const uint32_t mask = cones[cone].mask;
+    const int64_t base = cones[cone].fine_base;
+    const float* W0 = W;      // 6 x 3
+    const float* W1 = W + 18; // 9 x 6
+    const float* W2 = W + 72; // 9 x 6
+    const float a1a = c < 9 ? W1[c * 6 + g] : 0.f;
+    const float a1b = (c < 9 && g < 2) ? W1[c * 6 + g + 4] : 0.f;
+    const float a2a = c < 9 ? W2[c * 6 + g] : 0.f;
+    const float a2b = (c < 9 && g < 2) ? W2[c * 6 + g + 4] : 0.f;
+    const float a0 = (c < 6 && g < 3) ? W0[c * 3 + g] : 0.f;
+    const float* v = V + cone * (108 * NC) + n;
+    float b[18][2];
+#pragma unroll
+    for (int q = 0; q < 18; q++) // q = t0 + 3 t2
+    {
+        const int t0 = q % 3, t2 = q / 3;
+        b[q][0] = v[(t0 + 3 * (g + 6 * t2)) * NC];
+        b[q][1] = g < 2 ? v[(t0 + 3 * (g + 4 + 6 * t2)) * NC] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 18; q++)
+    {
+        const int t0 = q % 3, t2 = q / 3;
+        mf4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1a, b[q][0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1b, b[q][1], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+        {
+            const int j1 = 4 * g + i;
+            if (j1 < 9)
+                Y1[(t0 * 9 + j1) * S1 + t2 * 16 + c] = acc[i];
+        }
+    }
+    lds_fence();
+    for (int j1 = 0; j1 < 9; j1++)
+    {
+        const int s1 = j1 / 3, m1 = j1 - 3 * s1;
+        // sub-cones s0 + 3 s1 + 9 s2 for every s0, s2
+        if (!((mask >> (3 * s1)) & 0x1C0E07u))
+            continue;
+#pragma unroll
+        for (int t0 = 0; t0 < 3; t0++)
+        {
+            const float* y = Y1 + (t0 * 9 + j1) * S1 + c;
+            mf4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a2a, y[g * 16], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a2b, g < 2 ? y[(g + 4) * 16] : 0.f, acc,
+                                                       0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+            {
+                const int j2 = 4 * g + i;
+                if (j2 < 9)
+                    Y2[j2 * S2 + t0 * 16 + c] = acc[i];
+            }
+        }
+        lds_fence();
+        mf4 acc[9];
+#pragma unroll
+        for (int j2 = 0; j2 < 9; j2++)
+        {
+            acc[j2] = mf4{0.f, 0.f, 0.f, 0.f};
+            acc[j2] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                a0, g < 3 ? Y2[j2 * S2 + g * 16 + c] : 0.f, acc[j2], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j2 = 0; j2 < 9; j2++)
+        {
+            const int s2 = j2 / 3, m2 = j2 - 3 * s2;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+            {
+                const int j0 = 4 * g + i;
+                const int sub = (j0 >> 1) + 3 * s1 + 9 * s2;
+                if (j0 < 6 && ((mask >> sub) & 1u))
+                {
+                    const int rk = __popc(mask & ((1u << sub) - 1u));
+                    const int coef = (j0 & 1) + 2 * (m1 + 3 * m2);
+                    Fout[((base + rk) * 18 + coef) * NC + n] = acc[j2][i];
+                }
+            }
+        }
+        lds_fence();
+    }
+}
+
 template <typename R, int H0, int H1, int H2, int L0, int L1, int L2, int NI, int RB>
 void launch_transform_b(DevicePlan& D, hipStream_t st, int64_t c0, int64_t c1, const Cx<R>* V,
                         Cx<R>* F)
 {
     const R* W = static_cast<const R*>(D.wmat);
+    if constexpr (sizeof(R) == 4 && H0 == 3 && H1 == 6 && H2 == 6 && L0 == 2 && L1 == 3 &&
+                  L2 == 3)
+        if (std::getenv("IFGF_TRANSFORM_MFMA") != nullptr)
+        {
+            const int64_t blocks = (c1 - c0) * (2 * RB / 16);
+            k_transform_mf<RB><<<dim3(unsigned(blocks)), dim3(64), 0, st>>>(
+                c0, c1, D.cones, reinterpret_cast<const float*>(V), reinterpret_cast<float*>(F),
+                W, D.shape.xcd_log2);
+            return;
+        }
     const int64_t blocks = (c1 - c0) * RB / NI;
     k_transform_b<R, H0, H1, H2, L0, L1, L2, 3, 3, 3, NI, RB>
         <<<dim3(unsigned(blocks)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W,

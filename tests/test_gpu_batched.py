@@ -89,6 +89,80 @@ def test_batched_kernel_families_and_order8(gpu, prec):
     assert rel(Ut.cpu().numpy(), single_columns(g, W)) < TOL_SAME[prec]
 
 
+@pytest.mark.parametrize("nrhs", [5, 40])
+def test_mfma_transform_matches_separable_transform(gpu, nrhs):
+    """The batched transform on MFMA tiles (orders (3,6,6)/(2,3,3), mixed form, 8- and
+    32-column passes) against the separable VALU transform, the single-column mult and the
+    oracle; the gradient kernel shares the transform."""
+    x = spheroid(8000, 97)
+    W = np.stack([cdens(8000, 98 + i) for i in range(nrhs)], axis=1)
+    ops = []
+    for mfma in (False, True):
+        if mfma:
+            os.environ["IFGF_TRANSFORM_MFMA"] = "1"
+        try:
+            print(f"batched transform mfma={mfma} nrhs={nrhs}", flush=True)
+            op = ifgf_amd.HelmholtzIfgfOperator(12.0, 64, 6, 1, -1)
+            op.init(x.T, x.T)
+            ops.append((op, op.mult(W)))
+        finally:
+            os.environ.pop("IFGF_TRANSFORM_MFMA", None)
+    (op, U), (_, Um) = ops
+    o = op.plan_orders()
+    assert list(o["high"]) == [3, 6, 6] and list(o["low"]) == [2, 3, 3]
+    assert rel(Um, U) < TOL_SAME["mixed"]
+    ref = single_columns(op, W)
+    assert rel(Um, ref) < TOL_SAME["mixed"]
+    orc = po.Oracle(po.HELMHOLTZ, 12.0, 64, 6, 1, -1).init(x, x)
+    assert rel(Um[:, 0], orc.mult(W[:, 0])) < TOL_ORACLE["mixed"]
+    res = []
+    for mfma in (False, True):
+        if mfma:
+            os.environ["IFGF_TRANSFORM_MFMA"] = "1"
+        try:
+            g = ifgf_amd.GradHelmholtzIfgfOperator(12.0, 64, 6, 1, -1)
+            g.setDx(1)
+            g.init(x.T, x.T)
+            res.append(g.mult(W))
+        finally:
+            os.environ.pop("IFGF_TRANSFORM_MFMA", None)
+    assert rel(res[1], res[0]) < TOL_SAME["mixed"]
+
+
+def test_transform_forms_configuration3_spheroid(gpu):
+    """Configuration 3's spheroid (N=262,144, kappa=64, 32 columns, one stream) with the
+    separable VALU transform and with the MFMA transform: the same results to rounding; the
+    times per mult are printed (the kernel trace of this test compares the two kernels)."""
+    import torch
+    N = 262144
+    x = spheroid(N, 1003)
+    rng = np.random.default_rng(2003)
+    Wd = torch.as_tensor(rng.standard_normal((N, 32)) + 1j * rng.standard_normal((N, 32)),
+                         device="cuda:0")
+    out = {}
+    for mfma in (False, True):
+        if mfma:
+            os.environ["IFGF_TRANSFORM_MFMA"] = "1"
+        try:
+            op = ifgf_amd.HelmholtzIfgfOperator(64.0, 64, 6, 1, -1)
+            op.init(x.T, x.T)
+            op.launch_shape = dict(overlap=0)
+            op.mult(Wd)
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(3):
+                U = op.mult(Wd)
+            e.record()
+            torch.cuda.synchronize()
+            out[mfma] = U.cpu().numpy()
+            print(f"transform on {'MFMA' if mfma else 'VALU'}: {s.elapsed_time(e) / 3:.2f} ms "
+                  f"per 32-column mult", flush=True)
+        finally:
+            os.environ.pop("IFGF_TRANSFORM_MFMA", None)
+        del op
+    assert rel(out[True], out[False]) < TOL_SAME["mixed"]
+
+
 def test_configuration3_spheroid_32_columns(gpu):
     """N=262,144 oblate spheroid (1,1,0.5), kappa=64, 32 complex N(0,1) density columns in one
     batched pass (mixed form): columns against the single-column mult and against an FP64
