@@ -32,6 +32,8 @@ SIGNATURES = {
     "ifgf_create": (_I, [_I, _D, _D, _I64, _I, _I, _D, _I, ctypes.POINTER(_P)]),
     "ifgf_set_dx": (_I, [_P, _I]),
     "ifgf_init": (_I, [_P, _P, _I64, _P, _I64, _P, CUTOFF_FN, _P, _I]),
+    "ifgf_plan_save": (_I, [_P, ctypes.c_char_p]),
+    "ifgf_init_from_plan": (_I, [_P, ctypes.c_char_p, _I]),
     "ifgf_mult": (_I, [_P, _P, _I64, _P, _I, _P]),
     "ifgf_mult_noalloc": (_I, [_P, _P, _P, _P]),
     "ifgf_mult_adjoint": (_I, [_P, _P, _I64, _P, _I, _P]),

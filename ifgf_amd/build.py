@@ -25,7 +25,7 @@ def build(verbose=True):
     """Compile every translation unit (in parallel) and link libifgf_amd.so."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
-    for src in ("plan.cpp", "capi.cpp"):
+    for src in ("plan.cpp", "plan_io.cpp", "capi.cpp"):
         jobs.append((["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src)],
                      os.path.join(BUILD, src + ".o")))
     for src in ("device.hip", "batched.hip", "tree.hip", "cones.hip", "pairs.hip"):

@@ -10,6 +10,7 @@
 #include "../../include/ifgf_amd.h"
 #include "device.hpp"
 #include "plan.hpp"
+#include "plan_io.hpp"
 
 using namespace ifgf;
 
@@ -32,6 +33,9 @@ struct ifgf_operator
     bool host_plan = false; // octrees and cone marking on the host (IFGF_INIT_HOST_PLAN)
     ifgf_cutoff_fn cutoff = nullptr;
     void* cutoff_ctx = nullptr;
+    // the plan was read from a file and built with a cutOff callback that did not travel:
+    // a transposed plan (adjoint for other targets) cannot be rebuilt the same way
+    bool cutoff_lost = false;
     std::unique_ptr<ifgf_operator> adj;
 };
 
@@ -117,6 +121,7 @@ extern "C"
                 n_src == n_tgt && std::memcmp(srcs, tgts, sizeof(double) * 3 * n_src) == 0;
             op->cutoff = cutoff;
             op->cutoff_ctx = cutoff_ctx;
+            op->cutoff_lost = false;
             op->plan = Plan();
             op->plan.K = op->K;
             std::function<bool(double)> cb;
@@ -137,6 +142,102 @@ extern "C"
             if (!e.empty())
                 return fail(op, IFGF_EINTERNAL, e);
             op->on_device = true;
+        }
+        catch (const std::exception& ex)
+        {
+            return fail(op, IFGF_EINTERNAL, ex.what());
+        }
+        return IFGF_OK;
+    }
+
+    int ifgf_plan_save(ifgf_operator* op, const char* path)
+    {
+        if (!op || !path)
+            return IFGF_EINVAL;
+        if (!op->planned)
+            return fail(op, IFGF_ESTATE, "plan_save before init");
+        try
+        {
+            PlanFileInfo info;
+            info.leaf = op->leaf;
+            info.order = op->order;
+            info.n_elem = op->n_elem;
+            const LaunchShape& s = op->dev.shape;
+            const int sh[6] = {s.xcd_log2, s.cpb, s.up_k, s.rb, s.far_waves, s.overlap};
+            std::memcpy(info.shape, sh, sizeof(sh));
+            info.same_points = op->same_points;
+            info.merged = op->merged;
+            info.had_cutoff = op->cutoff != nullptr || op->cutoff_lost;
+            info.host_plan = op->host_plan;
+            std::string e = plan_save(op->plan, info, path);
+            if (!e.empty())
+                return fail(op, IFGF_EINVAL, e);
+        }
+        catch (const std::exception& ex)
+        {
+            return fail(op, IFGF_EINTERNAL, ex.what());
+        }
+        return IFGF_OK;
+    }
+
+    int ifgf_init_from_plan(ifgf_operator* op, const char* path, int flags)
+    {
+        if (!op || !path)
+            return IFGF_EINVAL;
+        const bool plan_only = (flags & IFGF_INIT_PLAN_ONLY) != 0;
+        if (!plan_only && !device_available())
+            return fail(op, IFGF_ENODEV, "no HIP device visible: ifgf_amd has no CPU path");
+        try
+        {
+            Plan P;
+            PlanFileInfo info;
+            std::string e = plan_load(P, info, path);
+            if (e.empty())
+            {
+                e = plan_check(P);
+                if (!e.empty())
+                    e = std::string(path) + " does not hold a consistent plan (" + e + ")";
+            }
+            if (!e.empty())
+                return fail(op, IFGF_EINVAL, e);
+            const KernelSpec& K = P.K;
+            if (K.kind != op->K.kind || K.k != op->K.k || K.kappa != op->K.kappa ||
+                K.tol != op->K.tol || info.leaf != op->leaf || info.order != op->order ||
+                info.n_elem != op->n_elem)
+                return fail(op, IFGF_EINVAL,
+                            std::string(path) +
+                                " holds the plan of an operator with another kernel, "
+                                "wavenumber, leaf size, order, n_elem or tolerance");
+            device_free(op->dev);
+            op->adj.reset();
+            op->on_device = false;
+            op->planned = false;
+            op->plan = std::move(P);
+            op->plan.K.dx = op->K.dx; // the direction is the operator's, not the plan's
+            op->same_points = info.same_points != 0;
+            op->merged = info.merged != 0;
+            op->host_plan = info.host_plan != 0;
+            op->cutoff = nullptr;
+            op->cutoff_ctx = nullptr;
+            op->cutoff_lost = info.had_cutoff != 0;
+            op->planned = true;
+            if (plan_only)
+                return IFGF_OK;
+            e = device_upload(op->plan, op->precision, op->dev,
+                              (flags & IFGF_INIT_LEVEL_RING) ? 1 : 0);
+            if (!e.empty())
+                return fail(op, IFGF_EINTERNAL, e);
+            op->on_device = true;
+            LaunchShape s;
+            s.xcd_log2 = info.shape[0];
+            s.cpb = info.shape[1];
+            s.up_k = info.shape[2];
+            s.rb = info.shape[3];
+            s.far_waves = info.shape[4];
+            s.overlap = info.shape[5];
+            e = device_set_shape(op->dev, s);
+            if (!e.empty())
+                return fail(op, IFGF_EINVAL, std::string(path) + ": launch shape: " + e);
         }
         catch (const std::exception& ex)
         {
@@ -199,6 +300,10 @@ extern "C"
             if (!op->same_points && op->merged)
                 return fail(op, IFGF_EINVAL,
                             "the adjoint of merged geometries needs targets = sources in each");
+            if (!op->same_points && op->cutoff_lost)
+                return fail(op, IFGF_EINVAL,
+                            "the adjoint for other targets rebuilds the plan with the cutOff "
+                            "callback, which a plan file does not hold: init this operator");
             if (!op->same_points)
             {
                 if (!op->adj)

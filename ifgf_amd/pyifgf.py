@@ -15,7 +15,7 @@ step in FP64).  mult also accepts a 2-D array (N, nrhs) and torch tensors on the
 (zero-copy, enqueued on torch's current stream).
 """
 import ctypes
-
+import os
 import sys
 
 import numpy as np
@@ -89,6 +89,26 @@ class _IfgfOperator:
                              | (_lib.INIT_LEVEL_RING if level_ring else 0))
         check(rc, self._h, "init")
         self._n, self._m = s.shape[0], t.shape[0]
+
+    def save_plan(self, path):
+        """Write this operator's plan (octrees, interaction lists, active cones, upward
+        tables) and launch shape to a file (not in the reference API, whose init rebuilds
+        them every time)."""
+        check(lib().ifgf_plan_save(self._h, os.fsencode(path)), self._h, "save_plan")
+
+    def load_plan(self, path, *, plan_only=False, level_ring=False):
+        """init from a file written by save_plan, for an operator of the same kernel,
+        wavenumber, leaf size, order, n_elem and tolerance: the same results, bit for bit,
+        without rebuilding the plan."""
+        rc = lib().ifgf_init_from_plan(self._h, os.fsencode(path),
+                                       (_lib.INIT_PLAN_ONLY if plan_only else 0)
+                                       | (_lib.INIT_LEVEL_RING if level_ring else 0))
+        check(rc, self._h, "load_plan")
+        n_src, n_tgt = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().ifgf_sizes(self._h, ctypes.byref(n_src), ctypes.byref(n_tgt)), self._h,
+              "sizes")
+        self._n, self._m = n_src.value, n_tgt.value
+        self._keep = None
 
     # IfgfOperator::mult (ifgfoperator.hpp:246-867)
     def mult(self, weights):

@@ -79,6 +79,24 @@ int ifgf_init(ifgf_operator* op, const double* srcs, int64_t n_src, const double
               int64_t n_tgt, const double* normals, ifgf_cutoff_fn cutoff, void* cutoff_ctx,
               int flags);
 
+/* Plan reuse across processes.  No counterpart in the reference, whose init rebuilds the
+ * octrees, interaction lists and cone domains every time (ifgfoperator.hpp:65-106,
+ * octree.hpp:202-816); these replace that work when the same points come back.
+ * ifgf_plan_save writes the operator's plan (both octrees, interaction lists, active cones,
+ * upward tables, the geometries of a merged operator) and its launch shape to a file.
+ * ifgf_init_from_plan reads one into an operator created with the same kernel, wavenumber,
+ * leaf size, order, n_elem and tolerance (IFGF_EINVAL otherwise), then uploads it as
+ * ifgf_init does (flags: IFGF_INIT_PLAN_ONLY, IFGF_INIT_LEVEL_RING) and applies the saved
+ * launch shape; the gradient direction stays the operator's (ifgf_set_dx).  Results equal
+ * those of the operator that wrote the file, bit for bit.  The file is checked (format,
+ * version, byte order, length, a 64-bit checksum, array sizes and indices); a damaged or
+ * foreign file gives IFGF_EINVAL and leaves the operator as it was.  Load only files this
+ * library wrote: the checks catch damage, not a file forged to pass them.  A plan built
+ * with a cutoff callback keeps its interaction lists, but the callback does not travel: the
+ * adjoint for targets other than the sources then needs ifgf_init. */
+int ifgf_plan_save(ifgf_operator* op, const char* path);
+int ifgf_init_from_plan(ifgf_operator* op, const char* path, int flags);
+
 /* IfgfOperator::mult (ifgfoperator.hpp:246-867) for nrhs density vectors.
  * stream: a hipStream_t (NULL = default stream).  With host pointers the call returns
  * after the result is copied back; with IFGF_DEVICE_POINTERS it only enqueues. */
