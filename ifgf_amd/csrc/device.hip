@@ -834,11 +834,21 @@ __global__ __launch_bounds__(256) void k_transform(int64_t c0, int64_t c1, int c
 // centre factor (ifgfoperator.hpp:962-1012, 1708-1891).  FAST (mixed form): cones located by
 // locate_fast with the level's fine-grid angle tables (doubles [ang_base, ang_base + ang_n) of
 // ang, shared by every source box of the level) staged in LDS; otherwise the exact locate.
+constexpr int FAR_STAGE = 4; // distinct cones per wave and source box staged in LDS
+
+// one-wave exchanges through LDS: the wave's LDS writes have completed before its next LDS
+// reads issue (other lanes read what a lane wrote); the memory clobber keeps the compiler from
+// moving LDS accesses across
+__device__ __forceinline__ void lds_wait()
+{
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+}
+
 template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int WPB>
 // the mixed order-6 kernel fits 64 VGPRs without spilling: ask for 8 waves per SIMD (the far
-// field is latency-bound; 74 registers would allow 6)
+// field is latency-bound; 74 registers would allow 6); at larger orders 4 waves (128 VGPRs)
 __global__ __launch_bounds__(64 * WPB)
-__attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) void k_far(const WorkItem* __restrict__ items,
+__attribute__((amdgpu_waves_per_eu(FAST && sizeof(R) == 4 ? (O1 == 3 ? 8 : 4) : 1))) void k_far(const WorkItem* __restrict__ items,
                                                  int64_t nitems,
                                             const int32_t* __restrict__ srange,
                                             const int32_t* __restrict__ far_src,
@@ -850,6 +860,7 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
                                             int ang_n, Cx<R>* __restrict__ result, KParams kp)
 {
     extern __shared__ double sang[];
+    __shared__ float4 fstage[WPB][FAST && O0 * O1 * O2 >= 50 ? FAR_STAGE * (O0 * O1 * O2 / 2) : 1];
     // WPB waves per workgroup, one item each, sharing the staged angle tables (one copy per
     // workgroup: the tables, not the registers, would otherwise bound the waves per CU)
     const int lane = WPB == 1 ? int(threadIdx.x) : int(threadIdx.x & 63);
@@ -919,16 +930,75 @@ __attribute__((amdgpu_waves_per_eu(FAST && O1 == 3 && sizeof(R) == 4 ? 8 : 1))) 
             cheb_t<O2>(x2, t2[0]);
             float2 v[1];
             constexpr int NC = O0 * O1 * O2;
+            constexpr int NQ = NC / 2; // float4 per cone
             float2 cc[NC];
-            const float4* p4 = reinterpret_cast<const float4*>(F + int64_t(fine) * NC);
-#pragma unroll
-            for (int i = 0; i < NC / 2; i++)
+            // at 50 and more coefficients the distinct cones of the wave's lanes (2.4 on
+            // average at 1M points; up to FAR_STAGE) are read from memory once, by all lanes
+            // together, into LDS, and every lane reads its cone there: the same values with
+            // a tenth of the vector-memory traffic.  A wave with more distinct cones, and
+            // every wave at fewer coefficients, reads memory directly.
+            bool staged = false;
+            if constexpr (NC >= 50)
             {
-                const float4 q = p4[i];
-                cc[2 * i] = make_float2(q.x, q.y);
-                cc[2 * i + 1] = make_float2(q.z, q.w);
+                const uint64_t act = __ballot(1);
+                const int rank = __popcll(act & ((1ull << lane) - 1ull));
+                const int nact = __popcll(act);
+                uint64_t rem = act;
+                int myslot = 0, nd = 0;
+                int cs[FAR_STAGE] = {};
+#pragma unroll 1
+                while (rem != 0 && nd < FAR_STAGE)
+                {
+                    const int l0 = __ffsll((unsigned long long)rem) - 1;
+                    const int f0 = __builtin_amdgcn_readlane(fine, l0);
+                    const uint64_t m = __ballot(fine == f0);
+                    if (fine == f0)
+                        myslot = nd;
+#pragma unroll
+                    for (int e = 0; e < FAR_STAGE; e++)
+                        cs[e] = nd == e ? f0 : cs[e];
+                    rem &= ~m;
+                    nd++;
+                }
+                if (rem == 0) // uniform
+                {
+                    float4* st4 = fstage[WPB == 1 ? 0 : (threadIdx.x >> 6)];
+                    for (int i = rank; i < nd * NQ; i += nact)
+                    {
+                        const int d = i / NQ, k = i - d * NQ;
+                        int cf = cs[0];
+#pragma unroll
+                        for (int e = 1; e < FAR_STAGE; e++)
+                            cf = d == e ? cs[e] : cf;
+                        st4[i] = reinterpret_cast<const float4*>(F + int64_t(cf) * NC)[k];
+                    }
+                    lds_wait(); // the wave's LDS writes are complete before it reads them
+                    const float4* q4 = st4 + myslot * NQ;
+#pragma unroll
+                    for (int i = 0; i < NQ; i++)
+                    {
+                        const float4 q = q4[i];
+                        cc[2 * i] = make_float2(q.x, q.y);
+                        cc[2 * i + 1] = make_float2(q.z, q.w);
+                    }
+                    eval_cone_g<O0, O1, O2, 1>(cc, t0, t1, t2, v);
+                    staged = true;
+                }
             }
-            eval_cone_g<O0, O1, O2, 1>(cc, t0, t1, t2, v);
+            if (!staged)
+            {
+                const float4* p4 = reinterpret_cast<const float4*>(F + int64_t(fine) * NC);
+#pragma unroll
+                for (int i = 0; i < NQ; i++)
+                {
+                    const float4 q = p4[i];
+                    cc[2 * i] = make_float2(q.x, q.y);
+                    cc[2 * i + 1] = make_float2(q.z, q.w);
+                }
+                eval_cone_g<O0, O1, O2, 1>(cc, t0, t1, t2, v);
+            }
+            // the next source box's staging writes stay behind this box's LDS reads
+            asm volatile("" ::: "memory");
             val = {v[0].x, v[0].y};
         }
         else
