@@ -78,12 +78,15 @@ template <typename R> __device__ __forceinline__ void sincos_r(R ph, R* s, R* c)
         sincos(ph, s, c);
 }
 
-// e^{-k delta} for a moderate argument (|k delta| of at most a few hundred radians)
-template <typename R> __device__ __forceinline__ Cx<R> expmk(double kr, double ki, R delta)
+// e^{-k delta} for a moderate argument (|k delta| of at most a few hundred radians).
+// DECAY = false: the caller's kernel family has a real wavenumber (k = -i kappa, kr = 0), and
+// the factor e^{-kr delta} is not formed at all (the same values: it is 1 there)
+template <typename R, bool DECAY = true>
+__device__ __forceinline__ Cx<R> expmk(double kr, double ki, R delta)
 {
     R s, c;
     sincos_r<R>(R(-ki) * delta, &s, &c);
-    if (kr != 0.0)
+    if (DECAY && kr != 0.0)
     {
         R m = exp(R(-kr) * delta);
         return {m * c, m * s};
@@ -92,8 +95,9 @@ template <typename R> __device__ __forceinline__ Cx<R> expmk(double kr, double k
 }
 
 // e^{-k r} with r in FP64; in the mixed form the phase is reduced modulo 2 pi in FP64
-// before the FP32 sincos
-template <typename R> __device__ __forceinline__ Cx<R> expmk_far(double kr, double ki, double r)
+// before the FP32 sincos (DECAY as for expmk)
+template <typename R, bool DECAY = true>
+__device__ __forceinline__ Cx<R> expmk_far(double kr, double ki, double r)
 {
     const double ph = -ki * r;
     R s, c;
@@ -106,7 +110,7 @@ template <typename R> __device__ __forceinline__ Cx<R> expmk_far(double kr, doub
     }
     else
         sincos(ph, &s, &c);
-    if (kr != 0.0)
+    if (DECAY && kr != 0.0)
     {
         R m;
         if constexpr (sizeof(R) == 4)
@@ -632,7 +636,7 @@ __device__ __forceinline__ Cx<R> p2p(const KParams& kp, R x0, R x1, R x2, double
             R s = -inv4pi * (invd * invd) * xd;
             return cmul(w, cscale(v, s));
         }
-        Cx<R> e = expmk<R>(kp.k_re, kp.k_im, d);
+        Cx<R> e = expmk<R, KIND != KH>(kp.k_re, kp.k_im, d);
         return cmul(w, cscale(e, inv4pi * invd));
     }
     else if constexpr (KIND == KL)
@@ -692,7 +696,7 @@ __device__ __forceinline__ Cx<R> factored(const KParams& kp, R y0, R y1, R y2, R
         delta = d - dc;
     if constexpr (KIND == KH || KIND == KM)
     {
-        Cx<R> e = expmk<R>(kp.k_re, kp.k_im, delta);
+        Cx<R> e = expmk<R, KIND != KH>(kp.k_re, kp.k_im, delta);
         return cmul(w, cscale(e, dc * invd));
     }
     else if constexpr (KIND == KL)

@@ -844,7 +844,7 @@ __device__ __forceinline__ void lds_wait()
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
 }
 
-template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int WPB>
+template <bool PH, typename R, int O0, int O1, int O2, bool FAST, int WPB, bool DEC>
 // the mixed order-6 kernel fits 64 VGPRs without spilling: ask for 8 waves per SIMD (the far
 // field is latency-bound; 74 registers would allow 6); at larger orders 4 waves (128 VGPRs)
 __global__ __launch_bounds__(64 * WPB)
@@ -1007,7 +1007,7 @@ __attribute__((amdgpu_waves_per_eu(FAST && sizeof(R) == 4 ? (O1 == 3 ? 8 : 4) : 
         if constexpr (!PH)
             cf = {R(INV_4PI_D * rinv), R(0)};
         else
-            cf = cscale(expmk_far<R>(B.k_re, B.k_im, r), R(INV_4PI_D * rinv));
+            cf = cscale(expmk_far<R, DEC>(B.k_re, B.k_im, r), R(INV_4PI_D * rinv));
         acc = cadd(acc, cmul(val, cf));
     };
     if (ngrp == 1)
@@ -1067,7 +1067,7 @@ static std::vector<WorkItem> make_items(const std::vector<int32_t>& boxes, const
 
 // up_level >= 0: the upward pass into parent level up_level; otherwise the far field of
 // target level far_level
-template <bool PH, typename R, int H0, int H1, int H2, int O0, int O1, int O2>
+template <bool PH, typename R, int H0, int H1, int H2, int O0, int O1, int O2, bool DEC>
 void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs,
                      int up_level, int far_level, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V,
                      Cx<R>* res)
@@ -1116,7 +1116,7 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
         const int abase = D.far_groups[far_level].ang_base, an = D.far_groups[far_level].ang_n;
         const int64_t n = b - a;
 #define IFGF_FAR(FAST, WPB, LDS, BASE, AN)                                                      \
-    k_far<PH, R, O0, O1, O2, FAST, WPB>                                                         \
+    k_far<PH, R, O0, O1, O2, FAST, WPB, DEC>                                                    \
         <<<dim3(unsigned((n + WPB - 1) / WPB)), dim3(64 * WPB), LDS, st>>>(                     \
             D.far_items + a, n, D.far_srange + 2 * a, D.far_src, D.geo, D.ctab, F, D.tgt_pts,   \
             D.fine_ang, BASE, AN, res, kp)
@@ -1145,20 +1145,22 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
     }
 }
 
-template <bool PH, typename R>
+// DEC: the kernel family's wavenumber may have a real part (e^{-k r} decays); false for the
+// real-kappa families, whose far field then forms the phase only
+template <bool PH, typename R, bool DEC>
 void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs, int up,
                    int far, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
 {
     switch (order_class(D))
     {
     case 1:
-        return launch_up_far_t<PH, R, 3, 6, 6, 2, 3, 3>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 3, 6, 6, 2, 3, 3, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 2:
-        return launch_up_far_t<PH, R, 5, 8, 8, 2, 5, 5>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 5, 8, 8, 2, 5, 5, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
     case 3:
-        return launch_up_far_t<PH, R, 6, 8, 7, 3, 5, 4>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 6, 8, 7, 3, 5, 4, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
     default:
-        return launch_up_far_t<PH, R, 0, 0, 0, 0, 0, 0>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 0, 0, 0, 0, 0, 0, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
     }
 }
 
@@ -1352,6 +1354,7 @@ template <int KIND, typename R>
 void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
 {
     constexpr bool PH = KIND != KL;
+    constexpr bool DEC = KIND != KH && KIND != KC; // real kappa: kr = 0
     const KParams kp = kparams(D);
     const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
     Cx<R>* ws = static_cast<Cx<R>*>(sg.ws);
@@ -1392,7 +1395,7 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
         [&](int g, hipStream_t s)
         {
             const DevicePlan::FarGroup& G = D.far_groups[g];
-            launch_up_far<PH, R>(D, s, kp, tabs, -1, g, G.item0, G.item1, Fl(G.src_level), V,
+            launch_up_far<PH, R, DEC>(D, s, kp, tabs, -1, g, G.item0, G.item1, Fl(G.src_level), V,
                                  res);
         });
     if (overlap != 1)
@@ -1423,7 +1426,7 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
         if (u1 > u0 || D.up_item_off[l + 1] > D.up_item_off[l])
         {
             pt.begin(2);
-            launch_up_far<PH, R>(D, st, kp, tabs, l, -1, u0, u1, Fl(l + 1), Vl(l), res);
+            launch_up_far<PH, R, DEC>(D, st, kp, tabs, l, -1, u0, u1, Fl(l + 1), Vl(l), res);
             pt.end();
         }
         if (c1 > c0)
