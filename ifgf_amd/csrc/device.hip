@@ -482,9 +482,10 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
 template <int O0, int O1, int O2, int NHI, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_up_grp(
     const WorkItem* __restrict__ items, int64_t nitems, const int32_t* __restrict__ glist,
-    const ConeDev* __restrict__ cones, const BoxGeo* __restrict__ geo,
-    const int32_t* __restrict__ child, const CellRef* __restrict__ ctab,
-    const UpGrp* __restrict__ grp, const int32_t* __restrict__ goff,
+    const int32_t* __restrict__ gcell, const ConeDev* __restrict__ cones,
+    const BoxGeo* __restrict__ geo, const int32_t* __restrict__ child,
+    const CellRef* __restrict__ ctab, const UpGrp* __restrict__ grp,
+    const int32_t* __restrict__ goff,
     const int64_t* __restrict__ part_cell, const int32_t* __restrict__ doff,
     const int32_t* __restrict__ dkey, const Cx<float>* __restrict__ F,
     Cx<float>* __restrict__ V, KParams kp)
@@ -503,11 +504,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         return;
     const WorkItem wi = items[it];
     const int tid = threadIdx.x;
-    const int64_t row = (part_cell[wi.pad] + wi.box) * 8;
 #pragma unroll 1
     for (int k = 0; k < wi.count; k++)
     {
         const int32_t g = glist[wi.first + k];
+        const int64_t row = (part_cell[wi.pad] + gcell[wi.first + k]) * 8;
         if (tid < 8)
         {
             const int32_t ch = child[8 * cones[g].box + tid];
@@ -1078,12 +1079,15 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
         const int64_t i0 = D.up_item_off[up_level], i1 = D.up_item_off[up_level + 1];
         if constexpr (sizeof(R) == 4 && O0 > 0)
         {
-            // FP32, compiled orders: the grouped tables, the cones of an item one at a time
-            if (i1 > i0)
+            // FP32, compiled orders: the grouped tables, the cones of an item one at a time,
+            // the items in tiles of boxes x cells
+            const int64_t t0 = D.up_tl_item_off[up_level], t1 = D.up_tl_item_off[up_level + 1];
+            if (t1 > t0)
                 k_up_grp<O0, O1, O2, H0 * H1 * H2, 128>
-                    <<<dim3(unsigned(i1 - i0)), dim3(128), 0, st>>>(
-                        D.up_items + i0, i1 - i0, D.up_glist + D.up_gl_off[up_level], D.cones,
-                        D.geo, D.child, D.ctab, static_cast<const UpGrp*>(D.up_grp), D.up_goff,
+                    <<<dim3(unsigned(t1 - t0)), dim3(128), 0, st>>>(
+                        D.up_tl_items + t0, t1 - t0, D.up_tl_glist + D.up_gl_off[up_level],
+                        D.up_tl_gcell + D.up_gl_off[up_level], D.cones, D.geo, D.child, D.ctab,
+                        static_cast<const UpGrp*>(D.up_grp), D.up_goff,
                         D.up_part_cell, D.up_doff, D.up_dkey, F, V, kp);
         }
         else if (i1 > i0)
@@ -1571,6 +1575,8 @@ void device_free(DevicePlan& D)
                     D.trig_ph, D.fine_ang, D.wmat, D.up_part_ent, D.up_entries, D.up_dkey,
                     D.up_doff, D.up_part_cell, D.up_goff, D.up_grp, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
+                    D.up_tl_items_k[0], D.up_tl_items_k[1], D.up_tl_items_k[2],
+                    D.up_tl_items_k[3], D.up_tl_glist, D.up_tl_gcell,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.F_bank[0], D.F_bank[1], D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in,
@@ -1939,12 +1945,15 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         // work lists.  Per level: leaf cones (leaf sampling), parent cones of boxes with a table
         // grouped by (part, cell) into items of at most UP_K_CHOICES[i] cones of one cell (boxes in order
         // within a cell), and the other parent cones (per-box upward pass)
-        std::vector<int32_t> leaf, up, glist;
-        std::vector<WorkItem> uitems[4];
+        std::vector<int32_t> leaf, up, glist, tl_glist, tl_gcell;
+        std::vector<WorkItem> uitems[4], tl_items[4];
         D.up_off.assign(D.levels + 1, 0);
         D.up_gl_off.assign(D.levels + 1, 0);
         for (int kc = 0; kc < 4; kc++)
+        {
             D.up_item_off_k[kc].assign(D.levels + 1, 0);
+            D.up_tl_item_off_k[kc].assign(D.levels + 1, 0);
+        }
         D.cone_lvl_off.assign(D.levels + 1, 0);
         for (int l = 0; l < D.levels; l++)
         {
@@ -1954,7 +1963,10 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
             D.up_off[l] = int64_t(up.size());
             D.up_gl_off[l] = int64_t(glist.size());
             for (int kc = 0; kc < 4; kc++)
+            {
                 D.up_item_off_k[kc][l] = int64_t(uitems[kc].size());
+                D.up_tl_item_off_k[kc][l] = int64_t(tl_items[kc].size());
+            }
             const int64_t g0 = int64_t(glist.size());
             for (int64_t g = b0; g < b1; g++)
                 for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
@@ -1991,6 +2003,38 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
                     uitems[kc].push_back({u, int32_t(i - g0), int32_t(j - i), pa});
                     i = j;
                 }
+            // the grouped kernel's list: tiles of boxes x cells (measured at 1M points,
+            // (6,8,7)/(3,5,4): 7.81 -> 7.4 ms for the upward phase, the same results)
+            const size_t h0 = tl_glist.size();
+            tl_glist.insert(tl_glist.end(), glist.begin() + g0, glist.end());
+            auto tb = [&](int32_t c) { return int64_t(P.cone_box[c] - b0) / UP_TILE_BOXES; };
+            auto tc = [&](int32_t c) { return cu(c) / UP_TILE_CELLS; };
+            std::stable_sort(tl_glist.begin() + h0, tl_glist.end(), [&](int32_t a, int32_t b)
+                             {
+                                 if (part(a) != part(b))
+                                     return part(a) < part(b);
+                                 if (tb(a) != tb(b))
+                                     return tb(a) < tb(b);
+                                 if (tc(a) != tc(b))
+                                     return tc(a) < tc(b);
+                                 if (cu(a) != cu(b))
+                                     return cu(a) < cu(b);
+                                 return cmask(a) < cmask(b);
+                             });
+            for (size_t i = h0; i < tl_glist.size(); i++)
+                tl_gcell.push_back(cu(tl_glist[i]));
+            for (int kc = 0; kc < 4; kc++)
+                for (size_t i = h0; i < tl_glist.size();)
+                {
+                    const int32_t a = tl_glist[i], pa = part(a);
+                    size_t j = i;
+                    while (j < tl_glist.size() && int(j - i) < UP_K_CHOICES[kc] &&
+                           part(tl_glist[j]) == pa && tb(tl_glist[j]) == tb(a) &&
+                           tc(tl_glist[j]) == tc(a))
+                        j++;
+                    tl_items[kc].push_back({cu(a), int32_t(i - h0), int32_t(j - i), pa});
+                    i = j;
+                }
         }
         D.up_off[D.levels] = int64_t(up.size());
         D.up_gl_off[D.levels] = int64_t(glist.size());
@@ -1998,10 +2042,16 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         {
             D.up_item_off_k[kc][D.levels] = int64_t(uitems[kc].size());
             D.up_items_k[kc] = dupload(uitems[kc], tot);
+            D.up_tl_item_off_k[kc][D.levels] = int64_t(tl_items[kc].size());
+            D.up_tl_items_k[kc] = dupload(tl_items[kc], tot);
         }
         D.up_items = D.up_items_k[3]; // the default shape: 16 cones per item
         D.up_item_off = D.up_item_off_k[3];
+        D.up_tl_items = D.up_tl_items_k[3];
+        D.up_tl_item_off = D.up_tl_item_off_k[3];
         D.up_glist = dupload(glist, tot);
+        D.up_tl_glist = dupload(tl_glist, tot);
+        D.up_tl_gcell = dupload(tl_gcell, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
         {
@@ -2364,6 +2414,8 @@ std::string device_set_shape(DevicePlan& D, const LaunchShape& sh)
     D.shape = sh;
     D.up_items = D.up_items_k[kc];
     D.up_item_off = D.up_item_off_k[kc];
+    D.up_tl_items = D.up_tl_items_k[kc];
+    D.up_tl_item_off = D.up_tl_item_off_k[kc];
     if (changed)
         device_drop_graph(D); // the captured kernels have the old shapes
     return "";

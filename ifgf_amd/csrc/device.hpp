@@ -72,6 +72,10 @@ struct LaunchShape
                        // field there too (1) or first on the main stream (2); 0: one stream
 };
 constexpr int UP_K_CHOICES[4] = {2, 4, 8, 16};
+// tiles of the grouped upward kernel's work list: parent boxes (in Morton order within a level)
+// x cells (table index) per tile
+constexpr int UP_TILE_BOXES = 16;
+constexpr int UP_TILE_CELLS = 128;
 
 // bytes of the fixed device scratch of the allocation-free mult (device_mult_noalloc): the
 // Morton-order densities and results of one column (64 MiB: n_src + n_tgt up to 8,388,576
@@ -135,6 +139,15 @@ struct DevicePlan
     std::vector<int64_t> up_item_off;        // per level, into up_items
     int32_t* up_glist = nullptr;             // the items' cones, level-major
     std::vector<int64_t> up_gl_off;          // per level, into up_glist
+    // the same cones for k_up_grp (which takes an item's cones one at a time): ordered by tiles
+    // of UP_TILE_BOXES boxes x UP_TILE_CELLS cells, items of at most UP_K_CHOICES[i] cones
+    // within a tile, each cone with its cell's table index (tl_gcell); same per-level offsets
+    WorkItem* up_tl_items_k[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<int64_t> up_tl_item_off_k[4];
+    WorkItem* up_tl_items = nullptr;
+    std::vector<int64_t> up_tl_item_off;
+    int32_t* up_tl_glist = nullptr;
+    int32_t* up_tl_gcell = nullptr;
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
