@@ -650,7 +650,10 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     }
     __syncthreads();
     // the axis maps at compile-time offsets from the kernel argument: scalar loads, the
-    // weights stay in scalar registers as FMA operands
+    // weights stay in scalar registers as FMA operands.  Maps too large for the scalar
+    // registers are read a row at a time (RW: a compiler barrier after each output keeps the
+    // next row's loads from being hoisted, which would spill scalar registers to vector lanes)
+    constexpr bool RW = sizeof(R) * (NW0 + NW1 + NW2) > 512;
     const R* W0 = W;
     const R* W1 = W + NW0;
     const R* W2 = W + NW0 + NW1;
@@ -673,6 +676,8 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
                 a.y += W0[j * H0 + t] * in[t].y;
             }
             Y[k * YS + p * M0 + j] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
         }
     }
     __syncthreads();
@@ -696,6 +701,8 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
                 a.y += W1[j * H1 + t] * in[t].y;
             }
             X[k * XS + j0 + M0 * (j + M1 * i2)] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
         }
     }
     __syncthreads();
@@ -731,6 +738,8 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
                 const int rk = __popc(mask & ((1u << sub) - 1u));
                 Fout[int64_t(base + rk) * NLO + m01 + L0 * L1 * m2] = a;
             }
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
         }
     }
 }
