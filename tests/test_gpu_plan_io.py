@@ -76,3 +76,38 @@ def test_cutoff_plan(gpu, tmp_path):
     assert np.array_equal(b.mult(w), a.mult(w))
     with pytest.raises(IfgfError, match="cutOff"):
         b.mult_adjoint(cdens(6000, 61))
+
+
+def test_merged_plan_and_allocation_free_mult(gpu, tmp_path):
+    """A merged operator's plan file loads into an operator of its first part's parameters and
+    gives the merged results (host and device densities); a loaded plan serves the
+    allocation-free mult with the same bits as its mult."""
+    import ctypes
+
+    import torch
+
+    from ifgf_amd import _lib
+    a = ifgf_amd.HelmholtzIfgfOperator(10.0, 64, 6, 1, -1)
+    b = ifgf_amd.HelmholtzIfgfOperator(16.0, 64, 6, 1, -1)
+    xa, xb = sphere(12000, 62), spheroid(9000, 63)
+    a.init(xa.T, xa.T)
+    b.init(xb.T, xb.T)
+    m = ifgf_amd.merge([a, b])
+    w = cdens(21000, 64)
+    u = m.mult(w)
+    m.save_plan(tmp_path / "m.plan")
+    print("merged plan file: 12,000 + 9,000 points", flush=True)
+    op = ifgf_amd.HelmholtzIfgfOperator(10.0, 64, 6, 1, -1)
+    op.load_plan(tmp_path / "m.plan")
+    assert np.array_equal(op.mult(w), u)
+    wd = torch.as_tensor(w, device="cuda:0")
+    ud = op.mult(wd)
+    torch.cuda.synchronize()
+    assert np.array_equal(ud.cpu().numpy(), u)
+    out = torch.empty_like(wd)
+    stream = torch.cuda.current_stream(wd.device).cuda_stream
+    rc = _lib.lib().ifgf_mult_noalloc(op._h, ctypes.c_void_p(wd.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+    assert rc == _lib.IFGF_OK
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), ud.cpu().numpy())
