@@ -473,13 +473,114 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
     }
 }
 
+// whether k_up_grp transforms its cones itself, by the number of fine coefficients: measured
+// at 1M points, a gain at (6,8,7)/(3,5,4) (60; 0.44 ms per mult) and a loss at (3,6,6)/(2,3,3)
+// (18; 0.1 ms)
+constexpr bool up_fuses_transform(int n_lo) { return n_lo >= 50; }
+
+// the values -> fine coefficients transform of one cone by NT threads (the arithmetic of
+// k_transform_t, pencil by pencil): X holds the cone's n_hi values on entry and is a work
+// buffer of max(n_hi, M0 M1 H2) entries, Y follows it (M0 H1 H2 entries); the coefficients of
+// the active sub-cones go to Fout from fine slot `base` on
+template <int H0, int H1, int H2, int L0, int L1, int L2, int NT>
+__device__ __forceinline__ void transform_one(Cx<float>* X, uint32_t mask, int base,
+                                              const float* __restrict__ W,
+                                              Cx<float>* __restrict__ Fout, int tid)
+{
+    constexpr int F0 = 3, F1 = 3, F2 = 3;
+    constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
+    constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2;
+    constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
+    constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
+    constexpr bool RW = sizeof(float) * (NW0 + NW1 + NW2) > 512;
+    Cx<float>* Y = X + XS;
+    const float* W0 = W;
+    const float* W1 = W + NW0;
+    const float* W2 = W + NW0 + NW1;
+    for (int p = tid; p < H1 * H2; p += NT)
+    {
+        Cx<float> in[H0];
+#pragma unroll
+        for (int t = 0; t < H0; t++)
+            in[t] = X[p * H0 + t];
+#pragma unroll
+        for (int j = 0; j < M0; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H0; t++)
+            {
+                a.x += W0[j * H0 + t] * in[t].x;
+                a.y += W0[j * H0 + t] * in[t].y;
+            }
+            Y[p * M0 + j] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < M0 * H2; p += NT)
+    {
+        const int j0 = p % M0, i2 = p / M0;
+        Cx<float> in[H1];
+#pragma unroll
+        for (int t = 0; t < H1; t++)
+            in[t] = Y[j0 + M0 * (t + H1 * i2)];
+#pragma unroll
+        for (int j = 0; j < M1; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H1; t++)
+            {
+                a.x += W1[j * H1 + t] * in[t].x;
+                a.y += W1[j * H1 + t] * in[t].y;
+            }
+            X[j0 + M0 * (j + M1 * i2)] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < M0 * M1; p += NT)
+    {
+        const int j0 = p % M0, j1 = p / M0;
+        const int s0 = j0 / L0, s1 = j1 / L1;
+        const int m01 = (j0 - s0 * L0) + L0 * (j1 - s1 * L1);
+        Cx<float> in[H2];
+#pragma unroll
+        for (int t = 0; t < H2; t++)
+            in[t] = X[p + M0 * M1 * t];
+#pragma unroll
+        for (int j = 0; j < M2; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H2; t++)
+            {
+                a.x += W2[j * H2 + t] * in[t].x;
+                a.y += W2[j * H2 + t] * in[t].y;
+            }
+            const int s2 = j / L2, m2 = j - s2 * L2;
+            const int sub = s0 + F0 * (s1 + F1 * s2);
+            if ((mask >> sub) & 1u)
+            {
+                const int rk = __popc(mask & ((1u << sub) - 1u));
+                Fout[int64_t(base + rk) * NLO + m01 + L0 * L1 * m2] = a;
+            }
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+}
+
 // Upward pass with the grouped tables (FP32, compiled orders; ifgfoperator.hpp:631-688,
 // 1639-1705).  A workgroup takes the parent cones of an item one at a time; all the cone's
 // (child, group) tasks run at once: a group is up to UP_GRP nodes falling in one child cone,
 // whose coefficients the thread reads once (from memory) for all of them, and writes each
 // node's value times its transfer factor to the node's partial sum of that child in LDS.  The
 // partial sums of every node are then added in child order, as the reference's.
-template <int O0, int O1, int O2, int NHI, int NT>
+template <int O0, int O1, int O2, int NHI, int NT, int H0, int H1, int H2, bool TF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_up_grp(
     const WorkItem* __restrict__ items, int64_t nitems, const int32_t* __restrict__ glist,
     const int32_t* __restrict__ gcell, const ConeDev* __restrict__ cones,
@@ -488,10 +589,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int32_t* __restrict__ goff,
     const int64_t* __restrict__ part_cell, const int32_t* __restrict__ doff,
     const int32_t* __restrict__ dkey, const Cx<float>* __restrict__ F,
-    Cx<float>* __restrict__ V, KParams kp)
+    Cx<float>* __restrict__ V, const float* __restrict__ W, Cx<float>* __restrict__ Fo,
+    KParams kp)
 {
     constexpr int NC = O0 * O1 * O2;
     static_assert(UP_GRP == 4, "records of four nodes");
+    static_assert(NHI == H0 * H1 * H2, "the high orders");
+    static_assert((H0 * H1 * H2 > 9 * O0 * O1 * H2 ? H0 * H1 * H2 : 9 * O0 * O1 * H2) +
+                          3 * O0 * H1 * H2 <=
+                      8 * NHI + 1,
+                  "the transform's buffers fit in part");
     constexpr int G = UP_GRP;
     constexpr int DMAX = 32;
     __shared__ int64_t toff[8];     // table offset of child c; -1: no child
@@ -597,17 +704,53 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         }
         __syncthreads();
-        for (int q = tid; q < NHI; q += NT)
+        if constexpr (!TF)
         {
-            float2 a = make_float2(0.0f, 0.0f);
-#pragma unroll
-            for (int c = 0; c < 8; c++)
+            for (int q = tid; q < NHI; q += NT)
             {
-                const float2 p = part[c * NHI + q];
-                a = make_float2(a.x + p.x, a.y + p.y);
+                float2 a = make_float2(0.0f, 0.0f);
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                {
+                    const float2 p = part[c * NHI + q];
+                    a = make_float2(a.x + p.x, a.y + p.y);
+                }
+                V[int64_t(g) * NHI + q] = Cx<float>{a.x, a.y};
             }
-            V[int64_t(g) * NHI + q] = Cx<float>{a.x, a.y};
+            __syncthreads(); // before the next cone replaces toff and part
+            continue;
         }
+        // TF: the node values (children in order), then straight on to the cone's fine
+        // coefficients: the values -> coefficients transform of k_transform_t for this one
+        // cone, its buffers in part's place (the values never go to memory)
+        constexpr int NQT = (NHI + NT - 1) / NT;
+        float2 sum[NQT];
+#pragma unroll
+        for (int i = 0; i < NQT; i++)
+        {
+            const int q = tid + i * NT;
+            float2 a = make_float2(0.0f, 0.0f);
+            if (q < NHI)
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                {
+                    const float2 p = part[c * NHI + q];
+                    a = make_float2(a.x + p.x, a.y + p.y);
+                }
+            sum[i] = a;
+        }
+        __syncthreads();
+        Cx<float>* X = reinterpret_cast<Cx<float>*>(part);
+#pragma unroll
+        for (int i = 0; i < NQT; i++)
+        {
+            const int q = tid + i * NT;
+            if (q < NHI)
+                X[q] = Cx<float>{sum[i].x, sum[i].y};
+        }
+        __syncthreads();
+        transform_one<H0, H1, H2, O0, O1, O2, NT>(X, cones[g].mask, cones[g].fine_base, W, Fo,
+                                                  tid);
         __syncthreads(); // before the next cone replaces toff and part
     }
 }
@@ -619,6 +762,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 template <typename R, int H0, int H1, int H2, int L0, int L1, int L2, int F0, int F1, int F2,
           int CPB>
 __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
+                                                     const int32_t* __restrict__ list,
                                                      const ConeDev* __restrict__ cones,
                                                      const Cx<R>* __restrict__ V,
                                                      Cx<R>* __restrict__ Fout,
@@ -635,18 +779,23 @@ __global__ __launch_bounds__(256) void k_transform_t(int64_t c0, int64_t c1,
     __shared__ Cx<R> Y[CPB * YS];
     __shared__ uint32_t cmask[CPB];
     __shared__ int cbase[CPB];
+    __shared__ int64_t cid[CPB];
     const int tid = threadIdx.x;
+    // cones [c0, c1), or with a list the cones list[c0 .. c1)
     const int64_t first = c0 + xcd_block(xlg) * CPB;
     const int ncone = int(min<int64_t>(CPB, c1 - first));
+    if (tid < ncone)
+    {
+        const int64_t c = list ? int64_t(list[first + tid]) : first + tid;
+        cid[tid] = c;
+        cmask[tid] = cones[c].mask;
+        cbase[tid] = cones[c].fine_base;
+    }
+    __syncthreads();
     for (int e = tid; e < ncone * NH; e += 256)
     {
         const int k = e / NH, i = e - k * NH;
-        X[k * XS + i] = V[(first + k) * NH + i];
-    }
-    if (tid < ncone)
-    {
-        cmask[tid] = cones[first + tid].mask;
-        cbase[tid] = cones[first + tid].fine_base;
+        X[k * XS + i] = V[cid[k] * NH + i];
     }
     __syncthreads();
     // the axis maps at compile-time offsets from the kernel argument: scalar loads, the
@@ -1080,7 +1229,7 @@ static std::vector<WorkItem> make_items(const std::vector<int32_t>& boxes, const
 template <bool PH, typename R, int H0, int H1, int H2, int O0, int O1, int O2, bool DEC>
 void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs,
                      int up_level, int far_level, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V,
-                     Cx<R>* res)
+                     Cx<R>* res, Cx<R>* Fo)
 {
     const bool up = up_level >= 0;
     if (up)
@@ -1091,13 +1240,17 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
             // FP32, compiled orders: the grouped tables, the cones of an item one at a time,
             // the items in tiles of boxes x cells
             const int64_t t0 = D.up_tl_item_off[up_level], t1 = D.up_tl_item_off[up_level + 1];
+            // (at the larger orders their transform included: the cones' fine coefficients
+            // go to Fo, the values nowhere)
+            constexpr bool TF = up_fuses_transform(O0 * O1 * O2);
             if (t1 > t0)
-                k_up_grp<O0, O1, O2, H0 * H1 * H2, 128>
+                k_up_grp<O0, O1, O2, H0 * H1 * H2, 128, H0, H1, H2, TF>
                     <<<dim3(unsigned(t1 - t0)), dim3(128), 0, st>>>(
                         D.up_tl_items + t0, t1 - t0, D.up_tl_glist + D.up_gl_off[up_level],
                         D.up_tl_gcell + D.up_gl_off[up_level], D.cones, D.geo, D.child, D.ctab,
                         static_cast<const UpGrp*>(D.up_grp), D.up_goff,
-                        D.up_part_cell, D.up_doff, D.up_dkey, F, V, kp);
+                        D.up_part_cell, D.up_doff, D.up_dkey, F, V,
+                        static_cast<const float*>(D.wmat), Fo, kp);
         }
         else if (i1 > i0)
         {
@@ -1162,18 +1315,19 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
 // real-kappa families, whose far field then forms the phase only
 template <bool PH, typename R, bool DEC>
 void launch_up_far(DevicePlan& D, hipStream_t st, const KParams& kp, const NodeTabs& tabs, int up,
-                   int far, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res)
+                   int far, int64_t a, int64_t b, Cx<R>* F, Cx<R>* V, Cx<R>* res,
+                   Cx<R>* Fo = nullptr)
 {
     switch (order_class(D))
     {
     case 1:
-        return launch_up_far_t<PH, R, 3, 6, 6, 2, 3, 3, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 3, 6, 6, 2, 3, 3, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res, Fo);
     case 2:
-        return launch_up_far_t<PH, R, 5, 8, 8, 2, 5, 5, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 5, 8, 8, 2, 5, 5, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res, Fo);
     case 3:
-        return launch_up_far_t<PH, R, 6, 8, 7, 3, 5, 4, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 6, 8, 7, 3, 5, 4, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res, Fo);
     default:
-        return launch_up_far_t<PH, R, 0, 0, 0, 0, 0, 0, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res);
+        return launch_up_far_t<PH, R, 0, 0, 0, 0, 0, 0, DEC>(D, st, kp, tabs, up, far, a, b, F, V, res, Fo);
     }
 }
 
@@ -1191,13 +1345,14 @@ static std::vector<int> transform_cpb_choices(int cls, size_t rsize)
 
 template <typename R>
 void launch_transform(DevicePlan& D, hipStream_t st, const KParams& kp, int64_t c0, int64_t c1,
-                      Cx<R>* V, Cx<R>* F)
+                      Cx<R>* V, Cx<R>* F, const int32_t* list = nullptr)
 {
     const R* W = static_cast<const R*>(D.wmat);
     const int cls = order_class(D);
 #define IFGF_TRANSFORM(H0, H1, H2, L0, L1, L2, CPB)                                            \
     k_transform_t<R, H0, H1, H2, L0, L1, L2, 3, 3, 3, CPB>                                      \
-        <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, D.cones, V, F, W, \
+        <<<dim3(unsigned((c1 - c0 + CPB - 1) / CPB)), dim3(256), 0, st>>>(c0, c1, list, D.cones, V, \
+                                                                          F, W,                 \
                                                                           kp.xlg)
     const int cpb_sel =
         D.shape.cpb > 0 ? D.shape.cpb : transform_cpb_choices(cls, sizeof(R))[0];
@@ -1368,6 +1523,11 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
 {
     constexpr bool PH = KIND != KL;
     constexpr bool DEC = KIND != KH && KIND != KC; // real kappa: kr = 0
+    // FP32 at the compiled orders with 50 and more fine coefficients: k_up_grp transforms the
+    // cones it computes
+    const int cls = order_class(D);
+    const bool fused = sizeof(R) == 4 && ((cls == 2 && up_fuses_transform(2 * 5 * 5)) ||
+                                          (cls == 3 && up_fuses_transform(3 * 5 * 4)));
     const KParams kp = kparams(D);
     const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
     Cx<R>* ws = static_cast<Cx<R>*>(sg.ws);
@@ -1436,17 +1596,26 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
                 D.normals, ws, tabs, Vl(l), kp);
             pt.end();
         }
+        if (c1 > c0)
+            pipe.reuse(l + 2); // level storage: F_bank[l & 1] held level l + 2
         if (u1 > u0 || D.up_item_off[l + 1] > D.up_item_off[l])
         {
             pt.begin(2);
-            launch_up_far<PH, R, DEC>(D, st, kp, tabs, l, -1, u0, u1, Fl(l + 1), Vl(l), res);
+            launch_up_far<PH, R, DEC>(D, st, kp, tabs, l, -1, u0, u1, Fl(l + 1), Vl(l), res,
+                                      Fl(l));
             pt.end();
         }
         if (c1 > c0)
         {
-            pipe.reuse(l + 2); // level storage: F_bank[l & 1] held level l + 2
             pt.begin(3);
-            launch_transform<R>(D, st, kp, c0, c1, Vl(l), Fl(l));
+            if (fused) // the grouped upward kernel has transformed its own cones
+            {
+                const int64_t t0 = D.tr_list_off[l], t1 = D.tr_list_off[l + 1];
+                if (t1 > t0)
+                    launch_transform<R>(D, st, kp, t0, t1, Vl(l), Fl(l), D.tr_list);
+            }
+            else
+                launch_transform<R>(D, st, kp, c0, c1, Vl(l), Fl(l));
             pt.end();
             pipe.fork();
         }
@@ -1585,7 +1754,7 @@ void device_free(DevicePlan& D)
                     D.up_doff, D.up_part_cell, D.up_goff, D.up_grp, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.up_tl_items_k[0], D.up_tl_items_k[1], D.up_tl_items_k[2],
-                    D.up_tl_items_k[3], D.up_tl_glist, D.up_tl_gcell,
+                    D.up_tl_items_k[3], D.up_tl_glist, D.up_tl_gcell, D.tr_list,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.F_bank[0], D.F_bank[1], D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in,
@@ -1954,7 +2123,8 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         // work lists.  Per level: leaf cones (leaf sampling), parent cones of boxes with a table
         // grouped by (part, cell) into items of at most UP_K_CHOICES[i] cones of one cell (boxes in order
         // within a cell), and the other parent cones (per-box upward pass)
-        std::vector<int32_t> leaf, up, glist, tl_glist, tl_gcell;
+        std::vector<int32_t> leaf, up, glist, tl_glist, tl_gcell, tr_list;
+        D.tr_list_off.assign(D.levels + 1, 0);
         std::vector<WorkItem> uitems[4], tl_items[4];
         D.up_off.assign(D.levels + 1, 0);
         D.up_gl_off.assign(D.levels + 1, 0);
@@ -1977,9 +2147,14 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
                 D.up_tl_item_off_k[kc][l] = int64_t(tl_items[kc].size());
             }
             const int64_t g0 = int64_t(glist.size());
+            D.tr_list_off[l] = int64_t(tr_list.size());
             for (int64_t g = b0; g < b1; g++)
                 for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
+                {
                     (P.S.leaf[g] ? leaf : (box_part[g] >= 0 ? glist : up)).push_back(int32_t(c));
+                    if (P.S.leaf[g] || box_part[g] < 0)
+                        tr_list.push_back(int32_t(c));
+                }
             auto part = [&](int32_t c) { return box_part[P.cone_box[c]]; };
             auto cu = [&](int32_t c) { return parts[part(c)]->cellmap[P.cone_cell[c]]; };
             // within a cell, cones of boxes with the same children next to one another: the
@@ -2061,6 +2236,8 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         D.up_glist = dupload(glist, tot);
         D.up_tl_glist = dupload(tl_glist, tot);
         D.up_tl_gcell = dupload(tl_gcell, tot);
+        D.tr_list_off[D.levels] = int64_t(tr_list.size());
+        D.tr_list = dupload(tr_list, tot);
         D.leaf_cones = dupload(leaf, tot);
         D.n_leaf_cones = int64_t(leaf.size());
         {

@@ -151,6 +151,10 @@ struct DevicePlan
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
+    // per level, the cones whose values -> coefficients transform runs on its own (leaf cones
+    // and cones without an upward table) when k_up_grp transforms the others itself
+    int32_t* tr_list = nullptr;
+    std::vector<int64_t> tr_list_off;
     WorkItem* leaf_boxes = nullptr; // leaf boxes with active cones: (box, first cone, count)
     int64_t n_leaf_boxes = 0;
     int32_t* up_cones = nullptr;             // non-leaf cones without a table, level-major
