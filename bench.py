@@ -82,14 +82,18 @@ def load_counters(tol):
         return None
 
 
-def flops_per_mult(work):
-    """Algorithmic flops of one mult (transcendentals count 1 each)."""
+def flops_per_mult(work, transform_cones=None):
+    """Algorithmic flops of one mult (transcendentals count 1 each).  transform_cones: the
+    cones of the standalone transform kernel, when the upward kernel transforms the others
+    itself (their transform flops then count for the upward phase)."""
     n_lo = work["n_lo"]
+    tc = work["coarse_cones"] if transform_cones is None else transform_cones
+    tf = work["transform_flops_per_cone"]
     return {
         "near": 23.0 * work["near_pairs"],
         "leaf": 26.0 * work["leaf_pairs"],
-        "upward": (54.0 + 4.0 * n_lo) * work["up_evals"],
-        "transform": work["transform_flops_per_cone"] * work["coarse_cones"],
+        "upward": (54.0 + 4.0 * n_lo) * work["up_evals"] + tf * (work["coarse_cones"] - tc),
+        "transform": tf * tc,
         "far": (44.0 + 4.0 * n_lo) * work["far_pairs"],
     }
 
@@ -275,6 +279,8 @@ def main():
     op.init(x.T, x.T)
     t_init = time.time() - t0
     work = op.plan_work()
+    # the cones of the standalone transform (fewer when the upward kernel transforms its own)
+    transform_cones = op.plan_counts()["transform_cones"]
     orders = {k: v.tolist() for k, v in op.plan_orders().items()}
     print(f"# init {t_init:.1f}s  {op.plan_counts()}  orders {orders}", file=sys.stderr,
           flush=True)
@@ -363,7 +369,7 @@ def main():
         del opf
         torch.cuda.empty_cache()
 
-    fl = flops_per_mult(work)
+    fl = flops_per_mult(work, transform_cones)
     full_size = (N, args.kappa, args.order, args.leaf) == (1048576, 100.0, 6, 64)
     counters = load_counters(args.tol) if full_size else None
     names = {"near": "near", "leaf": "leaf", "upward": "upward", "transform": "transform",

@@ -473,6 +473,7 @@ extern "C"
             c[13] += int64_t(pp.U.entries.size());
         }
         c[14] = op->dev.ring ? 1 : 0;
+        c[15] = op->on_device ? device_transform_cones(op->dev) : P.num_cones();
         return IFGF_OK;
     }
 

@@ -948,5 +948,17 @@ inline int order_class(const DevicePlan& D)
     return 0;
 }
 
+// whether k_up_grp transforms its cones itself, by the number of fine coefficients: measured
+// at 1M points, a gain at (6,8,7)/(3,5,4) (60; 0.44 ms per mult) and a loss at (3,6,6)/(2,3,3)
+// (18; 0.1 ms)
+constexpr bool up_fuses_transform(int n_lo) { return n_lo >= 50; }
+
+// the plan runs k_up_grp with its transform: FP32 at a compiled order class with enough fine
+// coefficients (the standalone transform then takes the per-level lists of the other cones)
+inline bool up_fused_transform(const DevicePlan& D)
+{
+    return D.prec != PREC_FP64 && order_class(D) != 0 && up_fuses_transform(D.n_lo);
+}
+
 } // namespace dev
 } // namespace ifgf

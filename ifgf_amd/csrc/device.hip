@@ -473,11 +473,6 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
     }
 }
 
-// whether k_up_grp transforms its cones itself, by the number of fine coefficients: measured
-// at 1M points, a gain at (6,8,7)/(3,5,4) (60; 0.44 ms per mult) and a loss at (3,6,6)/(2,3,3)
-// (18; 0.1 ms)
-constexpr bool up_fuses_transform(int n_lo) { return n_lo >= 50; }
-
 // the values -> fine coefficients transform of one cone by NT threads (the arithmetic of
 // k_transform_t, pencil by pencil): X holds the cone's n_hi values on entry and is a work
 // buffer of max(n_hi, M0 M1 H2) entries, Y follows it (M0 H1 H2 entries); the coefficients of
@@ -1523,11 +1518,8 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
 {
     constexpr bool PH = KIND != KL;
     constexpr bool DEC = KIND != KH && KIND != KC; // real kappa: kr = 0
-    // FP32 at the compiled orders with 50 and more fine coefficients: k_up_grp transforms the
-    // cones it computes
-    const int cls = order_class(D);
-    const bool fused = sizeof(R) == 4 && ((cls == 2 && up_fuses_transform(2 * 5 * 5)) ||
-                                          (cls == 3 && up_fuses_transform(3 * 5 * 4)));
+    // k_up_grp transforms the cones it computes (FP32, compiled orders, enough coefficients)
+    const bool fused = sizeof(R) == 4 && up_fused_transform(D);
     const KParams kp = kparams(D);
     const NodeTabs tabs{D.ho_nodes, D.trig_th, D.trig_ph};
     Cx<R>* ws = static_cast<Cx<R>*>(sg.ws);
@@ -1744,6 +1736,11 @@ bool device_available()
     if (hipGetDeviceCount(&n) != hipSuccess)
         return false;
     return n > 0;
+}
+
+int64_t device_transform_cones(const DevicePlan& D)
+{
+    return up_fused_transform(D) ? D.tr_list_off.back() : D.n_cones;
 }
 
 void device_free(DevicePlan& D)

@@ -168,7 +168,8 @@ class _IfgfOperator:
         check(lib().ifgf_plan_counts(self._h, c.ctypes.data), self._h, "plan_counts")
         keys = ["src_levels", "src_boxes", "tgt_levels", "tgt_boxes", "far_pairs", "near_pairs",
                 "coarse_cones", "fine_cones", "fine_uncovered", "dropped", "device_bytes",
-                "fine_ids", "up_table_levels", "up_table_entries", "level_ring"]
+                "fine_ids", "up_table_levels", "up_table_entries", "level_ring",
+                "transform_cones"]
         return {k: int(v) for k, v in zip(keys, c)}
 
     def plan_orders(self):

@@ -151,7 +151,10 @@ int ifgf_device_count(void);
  *         [8] fine cones active but not covered [9] dropped (cutOff) pairs
  *         [10] device bytes [11] fine cone ids listed
  *         [12] levels with precomputed upward geometry [13] entries of those tables
- *         [14] level storage (IFGF_INIT_LEVEL_RING) in use; the array holds 15 values */
+ *         [14] level storage (IFGF_INIT_LEVEL_RING) in use
+ *         [15] coarse cones of a mult's standalone values -> coefficients transform (fewer than
+ *              [6] when the upward pass transforms the cones it computes itself); the array
+ *              holds 16 values */
 int ifgf_plan_counts(const ifgf_operator* op, int64_t* counts);
 /* orders: base[3], coarse (high) order[3], fine (low) order[3], factor[3] */
 int ifgf_plan_orders(const ifgf_operator* op, int* out12);
