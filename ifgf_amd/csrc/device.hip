@@ -143,12 +143,112 @@ __global__ __launch_bounds__(64) void k_near(const WorkItem* __restrict__ items,
         result[t] = cadd(result[t], acc);
 }
 
+// the values -> fine coefficients transform of one cone by NT threads (the arithmetic of
+// k_transform_t, pencil by pencil): X holds the cone's n_hi values on entry and is a work
+// buffer of max(n_hi, M0 M1 H2) entries, Y follows it (M0 H1 H2 entries); the coefficients of
+// the active sub-cones go to Fout from fine slot `base` on
+template <int H0, int H1, int H2, int L0, int L1, int L2, int NT>
+__device__ __forceinline__ void transform_one(Cx<float>* X, uint32_t mask, int base,
+                                              const float* __restrict__ W,
+                                              Cx<float>* __restrict__ Fout, int tid)
+{
+    constexpr int F0 = 3, F1 = 3, F2 = 3;
+    constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
+    constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2;
+    constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
+    constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
+    constexpr bool RW = sizeof(float) * (NW0 + NW1 + NW2) > 512;
+    Cx<float>* Y = X + XS;
+    const float* W0 = W;
+    const float* W1 = W + NW0;
+    const float* W2 = W + NW0 + NW1;
+    for (int p = tid; p < H1 * H2; p += NT)
+    {
+        Cx<float> in[H0];
+#pragma unroll
+        for (int t = 0; t < H0; t++)
+            in[t] = X[p * H0 + t];
+#pragma unroll
+        for (int j = 0; j < M0; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H0; t++)
+            {
+                a.x += W0[j * H0 + t] * in[t].x;
+                a.y += W0[j * H0 + t] * in[t].y;
+            }
+            Y[p * M0 + j] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < M0 * H2; p += NT)
+    {
+        const int j0 = p % M0, i2 = p / M0;
+        Cx<float> in[H1];
+#pragma unroll
+        for (int t = 0; t < H1; t++)
+            in[t] = Y[j0 + M0 * (t + H1 * i2)];
+#pragma unroll
+        for (int j = 0; j < M1; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H1; t++)
+            {
+                a.x += W1[j * H1 + t] * in[t].x;
+                a.y += W1[j * H1 + t] * in[t].y;
+            }
+            X[j0 + M0 * (j + M1 * i2)] = a;
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < M0 * M1; p += NT)
+    {
+        const int j0 = p % M0, j1 = p / M0;
+        const int s0 = j0 / L0, s1 = j1 / L1;
+        const int m01 = (j0 - s0 * L0) + L0 * (j1 - s1 * L1);
+        Cx<float> in[H2];
+#pragma unroll
+        for (int t = 0; t < H2; t++)
+            in[t] = X[p + M0 * M1 * t];
+#pragma unroll
+        for (int j = 0; j < M2; j++)
+        {
+            Cx<float> a = {0, 0};
+#pragma unroll
+            for (int t = 0; t < H2; t++)
+            {
+                a.x += W2[j * H2 + t] * in[t].x;
+                a.y += W2[j * H2 + t] * in[t].y;
+            }
+            const int s2 = j / L2, m2 = j - s2 * L2;
+            const int sub = s0 + F0 * (s1 + F1 * s2);
+            if ((mask >> sub) & 1u)
+            {
+                const int rk = __popc(mask & ((1u << sub) - 1u));
+                Fout[int64_t(base + rk) * NLO + m01 + L0 * L1 * m2] = a;
+            }
+            if constexpr (RW)
+                asm volatile("" ::: "memory");
+        }
+    }
+}
+
 // leaf sampling (ifgfoperator.hpp:1057-1073, :423-462): the factored kernel of a leaf's
 // sources at the high-order nodes of its active coarse cones.  One 256-thread workgroup per
 // leaf box, thread = (active cone, high-order node) pairs of the box, 256 at a time, the box's
 // sources staged in LDS for each round: lanes stay busy however many cones the box has, and
 // the sources are staged once per 256 pairs
-template <int KIND, typename R>
+// H0 > 0 (FP32, the compiled orders H / L): the box's cones are then transformed to their fine
+// coefficients by the same workgroup (transform_one), their values read back from where this
+// workgroup has just written them (L2), instead of by the separate transform kernel
+template <int KIND, typename R, int H0 = 0, int H1 = 0, int H2 = 0, int L0 = 0, int L1 = 0,
+          int L2 = 0>
 __global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ items,
                                                   int64_t nitems,
                                                   const ConeDev* __restrict__ cones,
@@ -158,7 +258,9 @@ __global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ i
                                                   const double* __restrict__ spts,
                                                   const double* __restrict__ snrm,
                                                   const Cx<R>* __restrict__ w, NodeTabs tabs,
-                                                  Cx<R>* __restrict__ V, KParams kp)
+                                                  Cx<R>* __restrict__ V, KParams kp,
+                                                  const float* __restrict__ W = nullptr,
+                                                  Cx<float>* __restrict__ Fout = nullptr)
 {
     constexpr int CH = 256;
     __shared__ R sx[CH], sy[CH], sz[CH];
@@ -218,6 +320,24 @@ __global__ __launch_bounds__(256) void k_leaf_box(const WorkItem* __restrict__ i
         }
         if (active)
             V[int64_t(g) * kp.n_hi + q] = acc;
+    }
+    if constexpr (H0 > 0)
+    {
+        static_assert(sizeof(R) == 4, "the fused transform is FP32");
+        constexpr int NH = H0 * H1 * H2;
+        constexpr int XS = (NH > 9 * L0 * L1 * H2) ? NH : 9 * L0 * L1 * H2;
+        __shared__ Cx<float> T[XS + 3 * L0 * H1 * H2];
+        __syncthreads(); // the box's values written by this workgroup are visible to it
+        for (int k = 0; k < wi.count; k++)
+        {
+            const int32_t g = wi.first + k;
+            for (int i = int(threadIdx.x); i < NH; i += 256)
+                T[i] = V[int64_t(g) * NH + i];
+            __syncthreads();
+            transform_one<H0, H1, H2, L0, L1, L2, 256>(T, cones[g].mask, cones[g].fine_base, W,
+                                                       Fout, int(threadIdx.x));
+            __syncthreads();
+        }
     }
 }
 
@@ -470,102 +590,6 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
         // the next batch's address depends on this batch's sums: its loads follow them
         if (i2 % PB == PB - 1)
             asm volatile("" : "+v"(c) : "v"(out[0].x));
-    }
-}
-
-// the values -> fine coefficients transform of one cone by NT threads (the arithmetic of
-// k_transform_t, pencil by pencil): X holds the cone's n_hi values on entry and is a work
-// buffer of max(n_hi, M0 M1 H2) entries, Y follows it (M0 H1 H2 entries); the coefficients of
-// the active sub-cones go to Fout from fine slot `base` on
-template <int H0, int H1, int H2, int L0, int L1, int L2, int NT>
-__device__ __forceinline__ void transform_one(Cx<float>* X, uint32_t mask, int base,
-                                              const float* __restrict__ W,
-                                              Cx<float>* __restrict__ Fout, int tid)
-{
-    constexpr int F0 = 3, F1 = 3, F2 = 3;
-    constexpr int M0 = F0 * L0, M1 = F1 * L1, M2 = F2 * L2;
-    constexpr int NH = H0 * H1 * H2, NLO = L0 * L1 * L2;
-    constexpr int XS = (NH > M0 * M1 * H2) ? NH : M0 * M1 * H2;
-    constexpr int NW0 = M0 * H0, NW1 = M1 * H1, NW2 = M2 * H2;
-    constexpr bool RW = sizeof(float) * (NW0 + NW1 + NW2) > 512;
-    Cx<float>* Y = X + XS;
-    const float* W0 = W;
-    const float* W1 = W + NW0;
-    const float* W2 = W + NW0 + NW1;
-    for (int p = tid; p < H1 * H2; p += NT)
-    {
-        Cx<float> in[H0];
-#pragma unroll
-        for (int t = 0; t < H0; t++)
-            in[t] = X[p * H0 + t];
-#pragma unroll
-        for (int j = 0; j < M0; j++)
-        {
-            Cx<float> a = {0, 0};
-#pragma unroll
-            for (int t = 0; t < H0; t++)
-            {
-                a.x += W0[j * H0 + t] * in[t].x;
-                a.y += W0[j * H0 + t] * in[t].y;
-            }
-            Y[p * M0 + j] = a;
-            if constexpr (RW)
-                asm volatile("" ::: "memory");
-        }
-    }
-    __syncthreads();
-    for (int p = tid; p < M0 * H2; p += NT)
-    {
-        const int j0 = p % M0, i2 = p / M0;
-        Cx<float> in[H1];
-#pragma unroll
-        for (int t = 0; t < H1; t++)
-            in[t] = Y[j0 + M0 * (t + H1 * i2)];
-#pragma unroll
-        for (int j = 0; j < M1; j++)
-        {
-            Cx<float> a = {0, 0};
-#pragma unroll
-            for (int t = 0; t < H1; t++)
-            {
-                a.x += W1[j * H1 + t] * in[t].x;
-                a.y += W1[j * H1 + t] * in[t].y;
-            }
-            X[j0 + M0 * (j + M1 * i2)] = a;
-            if constexpr (RW)
-                asm volatile("" ::: "memory");
-        }
-    }
-    __syncthreads();
-    for (int p = tid; p < M0 * M1; p += NT)
-    {
-        const int j0 = p % M0, j1 = p / M0;
-        const int s0 = j0 / L0, s1 = j1 / L1;
-        const int m01 = (j0 - s0 * L0) + L0 * (j1 - s1 * L1);
-        Cx<float> in[H2];
-#pragma unroll
-        for (int t = 0; t < H2; t++)
-            in[t] = X[p + M0 * M1 * t];
-#pragma unroll
-        for (int j = 0; j < M2; j++)
-        {
-            Cx<float> a = {0, 0};
-#pragma unroll
-            for (int t = 0; t < H2; t++)
-            {
-                a.x += W2[j * H2 + t] * in[t].x;
-                a.y += W2[j * H2 + t] * in[t].y;
-            }
-            const int s2 = j / L2, m2 = j - s2 * L2;
-            const int sub = s0 + F0 * (s1 + F1 * s2);
-            if ((mask >> sub) & 1u)
-            {
-                const int rk = __popc(mask & ((1u << sub) - 1u));
-                Fout[int64_t(base + rk) * NLO + m01 + L0 * L1 * m2] = a;
-            }
-            if constexpr (RW)
-                asm volatile("" ::: "memory");
-        }
     }
 }
 
@@ -1513,6 +1537,37 @@ inline Stage plan_stage(const DevicePlan& D) { return {D.wsorted, D.result, true
 // everything of a mult between the gather of the densities and the scatter of the results:
 // with the plan's stage it reads and writes only the plan's buffers, so one captured graph
 // serves every call
+// leaf sampling of nb leaf boxes; fused: the kernel also transforms the boxes' cones to their
+// fine coefficients (in F)
+template <int KIND, typename R>
+void launch_leaf(DevicePlan& D, hipStream_t st, const WorkItem* boxes, int64_t nb, Cx<R>* ws,
+                 const NodeTabs& tabs, Cx<R>* V, Cx<R>* F, const KParams& kp, bool fused)
+{
+    if constexpr (sizeof(R) == 4)
+        if (fused)
+        {
+            const float* W = static_cast<const float*>(D.wmat);
+#define IFGF_LEAF_TF(H0, H1, H2, L0, L1, L2)                                                    \
+    k_leaf_box<KIND, R, H0, H1, H2, L0, L1, L2><<<dim3(unsigned(nb)), dim3(256), 0, st>>>(       \
+        boxes, nb, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts, D.normals, ws, tabs, V, kp, \
+        W, F)
+            switch (order_class(D))
+            {
+            case 2:
+                IFGF_LEAF_TF(5, 8, 8, 2, 5, 5);
+                return;
+            case 3:
+                IFGF_LEAF_TF(6, 8, 7, 3, 5, 4);
+                return;
+            default:
+                throw std::runtime_error("fused leaf transform for an order class without one");
+            }
+#undef IFGF_LEAF_TF
+        }
+    k_leaf_box<KIND, R><<<dim3(unsigned(nb)), dim3(256), 0, st>>>(
+        boxes, nb, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts, D.normals, ws, tabs, V, kp);
+}
+
 template <int KIND, typename R>
 void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
 {
@@ -1570,26 +1625,22 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
     {
         // every level's leaf cones at once (with level storage: a level at a time, below)
         pt.begin(1);
-        k_leaf_box<KIND, R><<<dim3(unsigned(D.n_leaf_boxes)), dim3(256), 0, st>>>(
-            D.leaf_boxes, D.n_leaf_boxes, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
-            D.normals, ws, tabs, V, kp);
+        launch_leaf<KIND, R>(D, st, D.leaf_boxes, D.n_leaf_boxes, ws, tabs, V, F, kp, fused);
         pt.end();
     }
     for (int l = D.levels - 1; l >= 0; l--)
     {
         const int64_t u0 = D.up_off[l], u1 = D.up_off[l + 1];
         const int64_t c0 = D.cone_lvl_off[l], c1 = D.cone_lvl_off[l + 1];
+        if (c1 > c0)
+            pipe.reuse(l + 2); // level storage: F_bank[l & 1] held level l + 2
         if (D.ring && D.leaf_lvl_off[l + 1] > D.leaf_lvl_off[l])
         {
             pt.begin(1);
             const int64_t a = D.leaf_lvl_off[l], nb = D.leaf_lvl_off[l + 1] - a;
-            k_leaf_box<KIND, R><<<dim3(unsigned(nb)), dim3(256), 0, st>>>(
-                D.leaf_boxes + a, nb, D.cones, D.geo, D.src_begin, D.src_end, D.src_pts,
-                D.normals, ws, tabs, Vl(l), kp);
+            launch_leaf<KIND, R>(D, st, D.leaf_boxes + a, nb, ws, tabs, Vl(l), Fl(l), kp, fused);
             pt.end();
         }
-        if (c1 > c0)
-            pipe.reuse(l + 2); // level storage: F_bank[l & 1] held level l + 2
         if (u1 > u0 || D.up_item_off[l + 1] > D.up_item_off[l])
         {
             pt.begin(2);
@@ -1600,7 +1651,7 @@ void run_core(DevicePlan& D, hipStream_t st, PhaseTimer& pt, const Stage& sg)
         if (c1 > c0)
         {
             pt.begin(3);
-            if (fused) // the grouped upward kernel has transformed its own cones
+            if (fused) // leaf sampling and the grouped upward kernel transformed their cones
             {
                 const int64_t t0 = D.tr_list_off[l], t1 = D.tr_list_off[l + 1];
                 if (t1 > t0)
@@ -2149,7 +2200,7 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
                 for (int64_t c = P.cone_off[g]; c < P.cone_off[g + 1]; c++)
                 {
                     (P.S.leaf[g] ? leaf : (box_part[g] >= 0 ? glist : up)).push_back(int32_t(c));
-                    if (P.S.leaf[g] || box_part[g] < 0)
+                    if (!P.S.leaf[g] && box_part[g] < 0)
                         tr_list.push_back(int32_t(c));
                 }
             auto part = [&](int32_t c) { return box_part[P.cone_box[c]]; };

@@ -151,8 +151,8 @@ struct DevicePlan
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
-    // per level, the cones whose values -> coefficients transform runs on its own (leaf cones
-    // and cones without an upward table) when k_up_grp transforms the others itself
+    // per level, the cones whose values -> coefficients transform runs on its own (non-leaf
+    // cones without an upward table) when leaf sampling and k_up_grp transform theirs
     int32_t* tr_list = nullptr;
     std::vector<int64_t> tr_list_off;
     WorkItem* leaf_boxes = nullptr; // leaf boxes with active cones: (box, first cone, count)
