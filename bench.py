@@ -84,15 +84,18 @@ def load_counters(tol):
 
 def flops_per_mult(work, transform_cones=None):
     """Algorithmic flops of one mult (transcendentals count 1 each).  transform_cones: the
-    cones of the standalone transform kernel, when the upward kernel transforms the others
-    itself (their transform flops then count for the upward phase)."""
+    cones of the standalone transform kernel; when it is fewer than all, leaf sampling and the
+    upward kernel transform their own cones, and those transform flops count for their phases
+    (the leaf cones for leaf sampling, the rest for the upward pass)."""
     n_lo = work["n_lo"]
-    tc = work["coarse_cones"] if transform_cones is None else transform_cones
+    cones = work["coarse_cones"]
+    tc = cones if transform_cones is None else transform_cones
     tf = work["transform_flops_per_cone"]
+    leaf_tf = work["leaf_cones"] if tc < cones else 0.0
     return {
         "near": 23.0 * work["near_pairs"],
-        "leaf": 26.0 * work["leaf_pairs"],
-        "upward": (54.0 + 4.0 * n_lo) * work["up_evals"] + tf * (work["coarse_cones"] - tc),
+        "leaf": 26.0 * work["leaf_pairs"] + tf * leaf_tf,
+        "upward": (54.0 + 4.0 * n_lo) * work["up_evals"] + tf * (cones - tc - leaf_tf),
         "transform": tf * tc,
         "far": (44.0 + 4.0 * n_lo) * work["far_pairs"],
     }

@@ -953,8 +953,9 @@ inline int order_class(const DevicePlan& D)
 // (18; 0.1 ms)
 constexpr bool up_fuses_transform(int n_lo) { return n_lo >= 50; }
 
-// the plan runs k_up_grp with its transform: FP32 at a compiled order class with enough fine
-// coefficients (the standalone transform then takes the per-level lists of the other cones)
+// leaf sampling and k_up_grp transform their cones: FP32 at a compiled order class with enough
+// fine coefficients (the standalone transform then takes the per-level lists of the other
+// cones)
 inline bool up_fused_transform(const DevicePlan& D)
 {
     return D.prec != PREC_FP64 && order_class(D) != 0 && up_fuses_transform(D.n_lo);

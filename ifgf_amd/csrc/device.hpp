@@ -230,7 +230,7 @@ struct DevicePlan
 std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring = 0);
 void device_free(DevicePlan& D);
 // coarse cones of one mult's standalone values -> coefficients transform (all of them unless
-// the grouped upward kernel transforms its own)
+// leaf sampling and the grouped upward kernel transform their own)
 int64_t device_transform_cones(const DevicePlan& D);
 // batched mult (batched.hip): ncol <= rb columns (rb = 8 or 32) of device densities with the
 // batch buffers allocated for rb; the compiled order classes only (batched_supported)

@@ -153,8 +153,8 @@ int ifgf_device_count(void);
  *         [12] levels with precomputed upward geometry [13] entries of those tables
  *         [14] level storage (IFGF_INIT_LEVEL_RING) in use
  *         [15] coarse cones of a mult's standalone values -> coefficients transform (fewer than
- *              [6] when the upward pass transforms the cones it computes itself); the array
- *              holds 16 values */
+ *              [6] when leaf sampling and the upward pass transform the cones they compute);
+ *              the array holds 16 values */
 int ifgf_plan_counts(const ifgf_operator* op, int64_t* counts);
 /* orders: base[3], coarse (high) order[3], fine (low) order[3], factor[3] */
 int ifgf_plan_orders(const ifgf_operator* op, int* out12);
