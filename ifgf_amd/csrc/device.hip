@@ -1188,15 +1188,30 @@ __attribute__((amdgpu_waves_per_eu(FAST && sizeof(R) == 4 ? (O1 == 3 ? 8 : 4) : 
             cf = cscale(expmk_far<R, DEC>(B.k_re, B.k_im, r), R(INV_4PI_D * rinv));
         acc = cadd(acc, cmul(val, cf));
     };
+    // the next source box's index is loaded one box ahead, so that its geometry loads do not
+    // wait for it
     if (ngrp == 1)
     {
-        for (int32_t j = j0; j < j1; j++) // uniform source box: scalar loads of its geometry
-            body(geo[__builtin_amdgcn_readfirstlane(far_src[j])]);
+        // uniform source box: scalar loads of its geometry
+        int32_t sn = j0 < j1 ? far_src[j0] : 0;
+        for (int32_t j = j0; j < j1; j++)
+        {
+            const int32_t sb = __builtin_amdgcn_readfirstlane(sn);
+            if (j + 1 < j1)
+                sn = far_src[j + 1];
+            body(geo[sb]);
+        }
     }
     else
     {
+        int32_t sn = j0 + grp < j1 ? far_src[j0 + grp] : 0;
         for (int32_t j = j0 + grp; j < j1; j += ngrp)
-            body(geo[far_src[j]]);
+        {
+            const int32_t sb = sn;
+            if (j + ngrp < j1)
+                sn = far_src[j + ngrp];
+            body(geo[sb]);
+        }
         for (int m = Tp; m < 64; m <<= 1)
         {
             acc.x += __shfl_xor(acc.x, m, 64);
