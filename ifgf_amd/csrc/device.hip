@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <type_traits>
 #include <utility>
 
@@ -601,15 +602,10 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
 // partial sums of every node are then added in child order, as the reference's.
 template <int O0, int O1, int O2, int NHI, int NT, int H0, int H1, int H2, bool TF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_up_grp(
-    const WorkItem* __restrict__ items, int64_t nitems, const int32_t* __restrict__ glist,
-    const int32_t* __restrict__ gcell, const ConeDev* __restrict__ cones,
-    const BoxGeo* __restrict__ geo, const int32_t* __restrict__ child,
+    const WorkItem* __restrict__ items, int64_t nitems, const UpDesc* __restrict__ desc,
     const CellRef* __restrict__ ctab, const UpGrp* __restrict__ grp,
-    const int32_t* __restrict__ goff,
-    const int64_t* __restrict__ part_cell, const int32_t* __restrict__ doff,
-    const int32_t* __restrict__ dkey, const Cx<float>* __restrict__ F,
-    Cx<float>* __restrict__ V, const float* __restrict__ W, Cx<float>* __restrict__ Fo,
-    KParams kp)
+    const Cx<float>* __restrict__ F, Cx<float>* __restrict__ V, const float* __restrict__ W,
+    Cx<float>* __restrict__ Fo, KParams kp)
 {
     constexpr int NC = O0 * O1 * O2;
     static_assert(UP_GRP == 4, "records of four nodes");
@@ -619,11 +615,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                       8 * NHI + 1,
                   "the transform's buffers fit in part");
     constexpr int G = UP_GRP;
-    constexpr int DMAX = 32;
-    __shared__ int64_t toff[8];     // table offset of child c; -1: no child
-    __shared__ int32_t tend[8];     // task list: child c's groups end at tend[c]
-    __shared__ int32_t tg0[8];      // record = tg0[c] + task index
-    __shared__ int32_t sslot[8][DMAX]; // slot of listed cone d of child c (-1: inactive)
+    constexpr int NW = int(sizeof(UpDesc) / 16);
+    // the cone's record: child tables, task ranges, the slots of the listed child cones
+    __shared__ UpDesc sd;
     __shared__ float2 part[8 * NHI + 1]; // node q's contribution from child c (+ a dump slot)
     const int64_t it = xcd_block(kp.xlg);
     if (it >= nitems)
@@ -633,60 +627,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll 1
     for (int k = 0; k < wi.count; k++)
     {
-        const int32_t g = glist[wi.first + k];
-        const int64_t row = (part_cell[wi.pad] + gcell[wi.first + k]) * 8;
-        if (tid < 8)
-        {
-            const int32_t ch = child[8 * cones[g].box + tid];
-            toff[tid] = ch >= 0 ? geo[ch].table_off : -1;
-        }
+        const int4* src = reinterpret_cast<const int4*>(desc + (wi.first + k));
+        for (int i = tid; i < NW; i += NT)
+            reinterpret_cast<int4*>(&sd)[i] = src[i];
         for (int i = tid; i < 8 * NHI; i += NT)
             part[i] = make_float2(0.0f, 0.0f);
         __syncthreads();
-        if (tid == 0)
-        {
-            int tot = 0;
-            for (int c = 0; c < 8; c++)
-            {
-                const int ng = toff[c] >= 0 ? goff[row + c + 1] - goff[row + c] : 0;
-                tg0[c] = goff[row + c] - tot;
-                tot += ng;
-                tend[c] = tot;
-            }
-        }
-        for (int i = tid; i < 8 * DMAX; i += NT)
-        {
-            const int c = i / DMAX, d = i - c * DMAX;
-            int32_t sl = -1;
-            if (toff[c] >= 0 && d < doff[row + c + 1] - doff[row + c])
-            {
-                const int kd = doff[row + c] + d;
-                const int32_t cf = dkey[2 * kd], sub = dkey[2 * kd + 1];
-                const CellRef cr = ctab[toff[c] + cf];
-                if ((cr.mask >> sub) & 1u)
-                    sl = cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u));
-            }
-            sslot[c][d] = sl;
-        }
-        __syncthreads();
-        const int tot = tend[7];
+        const int tot = sd.tend[7];
         for (int t = tid; t < tot; t += NT)
         {
             int c = 0;
-            while (t >= tend[c])
+            while (t >= sd.tend[c])
                 c++;
             // the record as six 16-byte loads: header, x0, x1, x2, tr, ti
-            const float4* ep = reinterpret_cast<const float4*>(grp + (tg0[c] + t));
+            const float4* ep = reinterpret_cast<const float4*>(grp + (sd.tg0[c] + t));
             const float4 hd = ep[0], ex0 = ep[1], ex1 = ep[2], ex2 = ep[3];
             const int32_t cf = __float_as_int(hd.x), subd = __float_as_int(hd.y);
             const int d = subd >> 8;
             int slot;
-            if (d < DMAX)
-                slot = sslot[c][d];
+            if (d < UP_DMAX)
+                slot = sd.slot[c][d];
             else
             {
                 const int sub = subd & 255;
-                const CellRef cr = ctab[toff[c] + cf];
+                const CellRef cr = ctab[sd.toff[c] + cf];
                 slot = ((cr.mask >> sub) & 1u) ? cr.fine_base + __popc(cr.mask & ((1u << sub) - 1u))
                                                : -1;
             }
@@ -734,9 +698,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const float2 p = part[c * NHI + q];
                     a = make_float2(a.x + p.x, a.y + p.y);
                 }
-                V[int64_t(g) * NHI + q] = Cx<float>{a.x, a.y};
+                V[int64_t(sd.g) * NHI + q] = Cx<float>{a.x, a.y};
             }
-            __syncthreads(); // before the next cone replaces toff and part
+            __syncthreads(); // before the next cone replaces sd and part
             continue;
         }
         // TF: the node values (children in order), then straight on to the cone's fine
@@ -768,9 +732,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 X[q] = Cx<float>{sum[i].x, sum[i].y};
         }
         __syncthreads();
-        transform_one<H0, H1, H2, O0, O1, O2, NT>(X, cones[g].mask, cones[g].fine_base, W, Fo,
-                                                  tid);
-        __syncthreads(); // before the next cone replaces toff and part
+        transform_one<H0, H1, H2, O0, O1, O2, NT>(X, sd.mask, sd.fine_base, W, Fo, tid);
+        __syncthreads(); // before the next cone replaces sd and part
     }
 }
 
@@ -1277,13 +1240,13 @@ void launch_up_far_t(DevicePlan& D, hipStream_t st, const KParams& kp, const Nod
             // (at the larger orders their transform included: the cones' fine coefficients
             // go to Fo, the values nowhere)
             constexpr bool TF = up_fuses_transform(O0 * O1 * O2);
+            if (t1 > t0 && !D.up_tl_desc)
+                throw std::runtime_error("upward pass: the per-cone records were not uploaded");
             if (t1 > t0)
                 k_up_grp<O0, O1, O2, H0 * H1 * H2, 128, H0, H1, H2, TF>
                     <<<dim3(unsigned(t1 - t0)), dim3(128), 0, st>>>(
-                        D.up_tl_items + t0, t1 - t0, D.up_tl_glist + D.up_gl_off[up_level],
-                        D.up_tl_gcell + D.up_gl_off[up_level], D.cones, D.geo, D.child, D.ctab,
-                        static_cast<const UpGrp*>(D.up_grp), D.up_goff,
-                        D.up_part_cell, D.up_doff, D.up_dkey, F, V,
+                        D.up_tl_items + t0, t1 - t0, D.up_tl_desc + D.up_gl_off[up_level],
+                        D.ctab, static_cast<const UpGrp*>(D.up_grp), F, V,
                         static_cast<const float*>(D.wmat), Fo, kp);
         }
         else if (i1 > i0)
@@ -1817,7 +1780,7 @@ void device_free(DevicePlan& D)
                     D.up_doff, D.up_part_cell, D.up_goff, D.up_grp, D.up_items_k[0],
                     D.up_items_k[1], D.up_items_k[2], D.up_items_k[3], D.up_glist,
                     D.up_tl_items_k[0], D.up_tl_items_k[1], D.up_tl_items_k[2],
-                    D.up_tl_items_k[3], D.up_tl_glist, D.up_tl_gcell, D.tr_list,
+                    D.up_tl_items_k[3], D.up_tl_glist, D.up_tl_gcell, D.up_tl_desc, D.tr_list,
                     D.leaf_cones, D.leaf_boxes, D.up_cones, D.far_srange, D.far_src,
                     D.near_off, D.near_src, D.far_items, D.near_items, D.wsorted, D.result, D.V,
                     D.F, D.F_bank[0], D.F_bank[1], D.b_ws, D.b_res, D.b_V, D.b_F, D.b_in,
@@ -2057,6 +2020,9 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
                 parts.push_back(&pp.U);
             box_part = P.box_up_part;
         }
+        // the FP32 tables' row offsets kept on the host for k_up_grp's per-cone records (below)
+        std::vector<int64_t> h_part_cell, h_goff;
+        std::vector<int32_t> h_doff, h_dkey;
         {
             std::vector<UpEntF> ef;
             std::vector<UpEntD> ed;
@@ -2177,6 +2143,10 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
                 std::vector<int32_t> goff32(goff.begin(), goff.end());
                 D.up_goff = dupload(goff32, tot);
                 D.up_grp = dupload(grp, tot);
+                h_part_cell = std::move(part_cell);
+                h_goff = std::move(goff);
+                h_doff = std::move(doff);
+                h_dkey = std::move(dkey);
             }
             if (precision == PREC_FP64)
                 D.up_entries = dupload(ed, tot);
@@ -2299,6 +2269,67 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         D.up_glist = dupload(glist, tot);
         D.up_tl_glist = dupload(tl_glist, tot);
         D.up_tl_gcell = dupload(tl_gcell, tot);
+        if (precision != PREC_FP64 && order_class(D) != 0 && !tl_glist.empty())
+        {
+            // k_up_grp's per-cone records, filled from the tables k_up_grp used to read per
+            // cone: children and their table offsets, the task ranges of each child's groups
+            // (the (cell, octant) rows of up_goff), the fine slots of the listed child cones
+            // (up_doff / up_dkey through the children's cell tables)
+            std::vector<UpDesc> desc(tl_glist.size());
+            auto fill = [&](size_t i0, size_t i1)
+            {
+                for (size_t i = i0; i < i1; i++)
+                {
+                    const int32_t g = tl_glist[i];
+                    const int64_t box = P.cone_box[g];
+                    const int64_t row = (h_part_cell[box_part[box]] + tl_gcell[i]) * 8;
+                    UpDesc& u = desc[i];
+                    u.g = g;
+                    u.mask = cones[g].mask;
+                    u.fine_base = cones[g].fine_base;
+                    u.pad = 0;
+                    int32_t tot_t = 0;
+                    for (int c = 0; c < 8; c++)
+                    {
+                        const int32_t ch = P.S.child[8 * box + c];
+                        const int64_t to = ch >= 0 ? geo[ch].table_off : -1;
+                        u.toff[c] = to;
+                        u.tg0[c] = int32_t(h_goff[row + c]) - tot_t;
+                        if (to >= 0)
+                            tot_t += int32_t(h_goff[row + c + 1] - h_goff[row + c]);
+                        u.tend[c] = tot_t;
+                        const int32_t nd = to >= 0 ? h_doff[row + c + 1] - h_doff[row + c] : 0;
+                        for (int d = 0; d < UP_DMAX; d++)
+                        {
+                            int32_t sl = -1;
+                            if (d < nd)
+                            {
+                                const int64_t kd = int64_t(h_doff[row + c]) + d;
+                                const int32_t cf = h_dkey[2 * kd], sub = h_dkey[2 * kd + 1];
+                                const int64_t tc = P.table[to + cf];
+                                if (tc >= 0)
+                                {
+                                    const uint32_t m = cones[tc].mask;
+                                    if ((m >> sub) & 1u)
+                                        sl = cones[tc].fine_base +
+                                             __builtin_popcount(m & ((1u << sub) - 1u));
+                                }
+                            }
+                            u.slot[c][d] = sl;
+                        }
+                    }
+                }
+            };
+            const size_t n = desc.size();
+            const size_t nth = std::min<size_t>(16, std::max<size_t>(1, n / 4096));
+            std::vector<std::thread> pool;
+            for (size_t t = 1; t < nth; t++)
+                pool.emplace_back(fill, n * t / nth, n * (t + 1) / nth);
+            fill(0, n / nth);
+            for (auto& th : pool)
+                th.join();
+            D.up_tl_desc = dupload(desc, tot);
+        }
         D.tr_list_off[D.levels] = int64_t(tr_list.size());
         D.tr_list = dupload(tr_list, tot);
         D.leaf_cones = dupload(leaf, tot);

@@ -77,6 +77,24 @@ constexpr int UP_K_CHOICES[4] = {2, 4, 8, 16};
 constexpr int UP_TILE_BOXES = 16;
 constexpr int UP_TILE_CELLS = 128;
 
+// What k_up_grp needs of a parent cone before its tasks, per entry of the tiled upward list
+// (FP32, compiled orders), computed at upload from the plan's fixed tables: read as one
+// 16-byte-aligned record instead of a chain of dependent loads (cone -> box -> children ->
+// their tables -> the (cell, octant) rows -> the listed child cones' slots).
+constexpr int UP_DMAX = 32; // listed child cones per (parent cone, child) with a slot here
+struct alignas(16) UpDesc
+{
+    int64_t toff[8];          // child c's coarse-cell table offset (-1: no child c)
+    int32_t tg0[8];           // the group record of child c's task t is tg0[c] + t
+    int32_t tend[8];          // child c's tasks end at tend[c] (prefix over the children)
+    int32_t g;                // the parent cone
+    uint32_t mask;            // its active fine sub-cones
+    int32_t fine_base;        // its first fine slot
+    int32_t pad;
+    int32_t slot[8][UP_DMAX]; // fine slot of listed child cone d of child c (-1: inactive)
+};
+static_assert(sizeof(UpDesc) == 1168, "UpDesc is 73 16-byte words");
+
 // bytes of the fixed device scratch of the allocation-free mult (device_mult_noalloc): the
 // Morton-order densities and results of one column (64 MiB: n_src + n_tgt up to 8,388,576
 // points in the mixed form, 4,194,272 in FP64)
@@ -148,6 +166,7 @@ struct DevicePlan
     std::vector<int64_t> up_tl_item_off;
     int32_t* up_tl_glist = nullptr;
     int32_t* up_tl_gcell = nullptr;
+    UpDesc* up_tl_desc = nullptr; // per entry of up_tl_glist (FP32, compiled orders)
 
     int32_t* leaf_cones = nullptr;
     int64_t n_leaf_cones = 0;
