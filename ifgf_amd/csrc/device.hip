@@ -595,7 +595,8 @@ __device__ __forceinline__ void eval_cone_g_mem(const float2* c,
 }
 
 // Upward pass with the grouped tables (FP32, compiled orders; ifgfoperator.hpp:631-688,
-// 1639-1705).  A workgroup takes the parent cones of an item one at a time; all the cone's
+// 1639-1705).  A workgroup takes the parent cones of an item one at a time, each from its
+// UpDesc record (read into LDS at once); all the cone's
 // (child, group) tasks run at once: a group is up to UP_GRP nodes falling in one child cone,
 // whose coefficients the thread reads once (from memory) for all of them, and writes each
 // node's value times its transfer factor to the node's partial sum of that child in LDS.  The
@@ -2271,10 +2272,10 @@ std::string device_upload(const Plan& P, int precision, DevicePlan& D, int ring)
         D.up_tl_gcell = dupload(tl_gcell, tot);
         if (precision != PREC_FP64 && order_class(D) != 0 && !tl_glist.empty())
         {
-            // k_up_grp's per-cone records, filled from the tables k_up_grp used to read per
-            // cone: children and their table offsets, the task ranges of each child's groups
-            // (the (cell, octant) rows of up_goff), the fine slots of the listed child cones
-            // (up_doff / up_dkey through the children's cell tables)
+            // k_up_grp's per-cone records, from the plan's fixed tables: the children and their
+            // cell-table offsets, the task ranges of each child's groups (the (cell, octant)
+            // rows of up_goff), the fine slots of the listed child cones (up_doff / up_dkey
+            // through the children's cell tables)
             std::vector<UpDesc> desc(tl_glist.size());
             auto fill = [&](size_t i0, size_t i1)
             {
